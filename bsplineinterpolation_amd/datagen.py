@@ -1,0 +1,116 @@
+"""Portable, seeded synthetic inputs for the benchmark workloads and tests.
+
+Counter-based SplitMix64: value ``i`` of stream ``s`` under ``seed`` is
+``mix(key(seed, s) + (i + 1) * 0x9E3779B97F4A7C15)``; uniforms take the top
+53 bits, Gaussians use Box-Muller on consecutive uniform pairs.  Any point
+of any stream can be regenerated independently, in any language, which the
+fixtures and the CPU baseline rely on.  ``std::*_distribution`` is never
+used: its output is implementation-defined.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SEED = 0x20261202
+STREAM_NOISE = 0
+STREAM_QUERY = 1
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+_SKEY = np.uint64(0xD1B54A32D192ED03)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
+    """``count`` doubles in [0, 1) from positions offset .. offset+count-1."""
+    with np.errstate(over="ignore"):
+        key = _mix(np.array([np.uint64(seed & 0xFFFFFFFFFFFFFFFF) ^ (np.uint64(stream) * _SKEY)],
+                            dtype=np.uint64))[0]
+        out = np.empty(count, dtype=np.float64)
+        chunk = 1 << 22
+        for b in range(0, count, chunk):
+            e = min(count, b + chunk)
+            i = np.arange(offset + b + 1, offset + e + 1, dtype=np.uint64)
+            z = _mix(key + i * _GOLD)
+            out[b:e] = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return out
+
+
+def normal(seed: int, stream: int, count: int, sigma: float = 1.0) -> np.ndarray:
+    """Box-Muller: pair j uses uniforms 2j, 2j+1 of the stream."""
+    m = (count + 1) // 2
+    u = uniform(seed, stream, 2 * m)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2.0 * math.pi * u2)
+    z[1::2] = r * np.sin(2.0 * math.pi * u2)
+    return sigma * z[:count]
+
+
+def grid_axis(lo: float, hi: float, n: int, periodic: bool) -> np.ndarray:
+    """Sample sites: n points on [lo, hi] (clamped) or n points of [lo, hi)
+    with spacing (hi - lo)/n (periodic, INTP_PERIODIC_NO_DUMMY_POINT)."""
+    if periodic:
+        return lo + (hi - lo) * np.arange(n) / n
+    return lo + (hi - lo) * np.arange(n) / (n - 1)
+
+
+def queries(seed: int, lo, hi, count: int, stream: int = STREAM_QUERY) -> np.ndarray:
+    """``count`` points, uniform in the box [lo, hi); SoA array [D, count].
+    Coordinate d of point i uses position d*count + i of the stream."""
+    D = len(lo)
+    u = uniform(seed, stream, D * count).reshape(D, count)
+    for d in range(D):
+        u[d] = lo[d] + (hi[d] - lo[d]) * u[d]
+    return u
+
+
+# ---------------------------------------------------------------------------
+# the named workloads
+# ---------------------------------------------------------------------------
+
+def workload_2d_64_cubic(seed: int = SEED, nq: int = 1_000_000):
+    """2D order-3 clamped FP64 on a 64x64 mesh over [0,1]^2,
+    f = sin(2 pi x) cos(3 pi y) + N(0, 1e-2^2); nq uniform queries."""
+    x = grid_axis(0.0, 1.0, 64, False)
+    y = grid_axis(0.0, 1.0, 64, False)
+    f = np.sin(2 * math.pi * x)[:, None] * np.cos(3 * math.pi * y)[None, :]
+    f = f + normal(seed, STREAM_NOISE, f.size, 1e-2).reshape(f.shape)
+    q = queries(seed, [0.0, 0.0], [1.0, 1.0], nq)
+    return dict(order=3, f=f, periodic=[False, False], ranges=[(0.0, 1.0), (0.0, 1.0)], queries=q,
+                orders=[[0, 0], [1, 0]])
+
+
+def workload_3d_256_cubic(seed: int = SEED, n: int = 256, nq: int = 1 << 26, nq_gen: int = None):
+    """3D order-3 FP64 on n^3 over [0,2pi) x [0,1] x [0,1], dim 0 periodic,
+    f = sin x exp(-(y-1/2)^2/0.05) cos(4 pi z) + N(0, 1e-3^2); nq uniform
+    queries for value and the three first partials.  ``nq_gen`` (<= nq)
+    generates only the leading queries (same values as the full set)."""
+    x = grid_axis(0.0, 2 * math.pi, n, True)
+    y = grid_axis(0.0, 1.0, n, False)
+    z = grid_axis(0.0, 1.0, n, False)
+    f = (np.sin(x)[:, None, None] * np.exp(-((y - 0.5) ** 2) / 0.05)[None, :, None]
+         * np.cos(4 * math.pi * z)[None, None, :])
+    f = f + normal(seed, STREAM_NOISE, f.size, 1e-3).reshape(f.shape)
+    lo, hi = [0.0, 0.0, 0.0], [2 * math.pi, 1.0, 1.0]
+    m = nq if nq_gen is None else nq_gen
+    u = uniform(seed, STREAM_QUERY, 3 * nq if nq_gen is None else 0)
+    if nq_gen is None:
+        q = u.reshape(3, nq)
+        for d in range(3):
+            q[d] = lo[d] + (hi[d] - lo[d]) * q[d]
+    else:
+        q = np.empty((3, m))
+        for d in range(3):
+            q[d] = lo[d] + (hi[d] - lo[d]) * uniform(seed, STREAM_QUERY, m, offset=d * nq)
+    return dict(order=3, f=f, periodic=[True, False, False], ranges=[(0.0, 2 * math.pi), (0.0, 1.0), (0.0, 1.0)],
+                queries=q, orders=[[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
