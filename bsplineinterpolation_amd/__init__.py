@@ -1,0 +1,282 @@
+"""MI355X-native tensor-product B-spline interpolation.
+
+Python mirror of the reference's class surface
+(12ff54e/BSplineInterpolation: intp::InterpolationFunction,
+InterpolationFunctionTemplate, BSpline) over the C ABI of libintp_amd.so
+(include/intp_amd.h).  Every build and evaluation runs the gfx950 kernels;
+host (numpy) inputs are staged to the device by the library, device
+(torch) tensors are used in place.  The C++ drop-in header
+include/intp/Interpolation.hpp offers the same surface to C++ callers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import DomainError, IntpError, check  # noqa: F401
+
+__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "DomainError", "IntpError"]
+
+_dp = C.POINTER(C.c_double)
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.split(".")[0] == "torch"
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def _current_stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Context:
+    """Binds one GPU (intp_ctx_create)."""
+
+    _defaults = {}
+
+    def __init__(self, device: int = 0):
+        lib = _lib.load()
+        h = C.c_void_p()
+        check(lib.intp_ctx_create(int(device), C.byref(h)))
+        self.h = h
+        self.device = int(device)
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        if device not in cls._defaults:
+            cls._defaults[device] = cls(device)
+        return cls._defaults[device]
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.intp_ctx_destroy(self.h)
+            self.h = None
+
+
+def _grid_desc(order: int, shape: Sequence[int], periodic, ranges, coords):
+    D = len(shape)
+    if not 1 <= D <= _lib.MAX_DIM:
+        raise IntpError(_lib.INTP_E_UNSUPPORTED, "dimension %d has no compiled kernels" % D)
+    desc = _lib.GridDesc()
+    desc.dim = D
+    desc.order = int(order)
+    periodic = list(periodic) if periodic is not None else [False] * D
+    coords = list(coords) if coords is not None else [None] * D
+    ranges = list(ranges) if ranges is not None else [None] * D
+    keep = []
+    for d in range(D):
+        desc.periodic[d] = int(bool(periodic[d]))
+        desc.n[d] = int(shape[d])
+        if coords[d] is not None:
+            c = np.ascontiguousarray(coords[d], dtype=np.float64)
+            if c.size != int(shape[d]) + int(bool(periodic[d])):
+                raise IntpError(_lib.INTP_E_INVALID,
+                                "Inconsistency between knot number and interpolated value number at dimension %d" % d)
+            keep.append(c)
+            desc.coords[d] = c.ctypes.data_as(_dp)
+        else:
+            lo, hi = ranges[d]
+            desc.xmin[d] = float(lo)
+            desc.xmax[d] = float(hi)
+    return desc, keep
+
+
+class InterpolationFunction:
+    """intp::InterpolationFunction<double, D, order> (Interpolation.hpp:17-507).
+
+    ``f`` is a C-order array of values (numpy: host, torch: device).
+    ``ranges[d]`` = (xmin, xmax) for a uniform dimension, or ``coords[d]`` =
+    node coordinates for a non-uniform one (n[d] + periodic[d] of them).
+    """
+
+    def __init__(self, order: int, f, periodic: Optional[Sequence[bool]] = None, ranges=None, coords=None,
+                 ctx: Optional[Context] = None, stream=None):
+        lib = _lib.load()
+        self.ctx = ctx or Context.default()
+        desc, keep = _grid_desc(order, tuple(f.shape), periodic, ranges, coords)
+        h = C.c_void_p()
+        if _is_torch(f):
+            f = f.contiguous()
+            assert str(f.dtype) == "torch.float64", "values must be float64"
+            st = _stream_handle(stream) if stream is not None else _current_stream()
+            check(lib.intp_build(self.ctx.h, C.byref(desc), C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE,
+                                 C.byref(h), st))
+        else:
+            fa = np.ascontiguousarray(f, dtype=np.float64)
+            check(lib.intp_build(self.ctx.h, C.byref(desc), fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST,
+                                 C.byref(h), _stream_handle(stream)))
+        del keep
+        self._set_handle(h)
+
+    @classmethod
+    def _wrap(cls, h, ctx):
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self._set_handle(h)
+        return self
+
+    def _set_handle(self, h):
+        self.h = h
+        info = _lib.SplineInfo()
+        check(_lib.load().intp_spline_get_info(h, C.byref(info)))
+        self.dim = info.dim
+        self.order = info.order
+        self._info = info
+
+    @classmethod
+    def from_knots(cls, order: int, knots: Sequence[np.ndarray], ctrl: np.ndarray, periodic=None,
+                   ctx: Optional[Context] = None) -> "InterpolationFunction":
+        """BSpline<double, D, order>(periodicity, ctrl, knot ranges...)."""
+        lib = _lib.load()
+        ctx = ctx or Context.default()
+        ctrl = np.ascontiguousarray(ctrl, dtype=np.float64)
+        D = ctrl.ndim
+        per = (C.c_int32 * _lib.MAX_DIM)(*[int(bool(p)) for p in (periodic or [False] * D)])
+        n = (C.c_int64 * _lib.MAX_DIM)(*ctrl.shape)
+        ks = [np.ascontiguousarray(k, dtype=np.float64) for k in knots]
+        kp = (_dp * _lib.MAX_DIM)(*[k.ctypes.data_as(_dp) for k in ks])
+        nk = (C.c_int64 * _lib.MAX_DIM)(*[k.size for k in ks])
+        h = C.c_void_p()
+        check(lib.intp_spline_from_knots(ctx.h, D, int(order), per, n, kp, nk, ctrl.ctypes.data_as(_dp),
+                                         C.byref(h)))
+        return cls._wrap(h, ctx)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.intp_spline_release(self.h)
+            self.h = None
+
+    # ---- properties (Interpolation.hpp:248-267) ----------------------
+    def periodicity(self, d: int) -> bool:
+        return bool(self._info.periodic[d])
+
+    def uniform(self, d: int) -> bool:
+        return bool(self._info.uniform[d])
+
+    def range(self, d: int):
+        return (self._info.range_lo[d], self._info.range_hi[d])
+
+    def get_order(self) -> int:
+        return self.order
+
+    def control_points(self) -> np.ndarray:
+        shape = tuple(self._info.n[d] for d in range(self.dim))
+        out = np.empty(shape)
+        check(_lib.load().intp_spline_download_ctrl(self.h, out.ctypes.data_as(_dp), out.size))
+        return out
+
+    def knots(self, d: int) -> np.ndarray:
+        out = np.empty(self._info.nknots[d])
+        check(_lib.load().intp_spline_download_knots(self.h, d, out.ctypes.data_as(_dp), out.size))
+        return out
+
+    # ---- evaluation ---------------------------------------------------
+    def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None):
+        """Batched evaluation.  ``points``: numpy [npts, D] (host) or a
+        float64 torch tensor [D, npts] (SoA) / [npts, D] (AoS) on the
+        device.  ``orders``: [nout, D] derivative orders (default: value).
+        Returns [nout, npts] (numpy or torch, like ``points``)."""
+        lib = _lib.load()
+        D = self.dim
+        orders = np.zeros((1, D), np.int32) if orders is None else \
+            np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
+        nout = orders.shape[0]
+        flags = _lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0
+        op = orders.ctypes.data_as(C.POINTER(C.c_int32))
+        if _is_torch(points):
+            import torch
+            assert points.dtype == torch.float64 and points.is_cuda
+            pts = points.contiguous()
+            if pts.dim() == 1 and D == 1:
+                pts = pts.reshape(1, -1)
+            if pts.shape[0] == D and (pts.dim() == 2 and (pts.shape[1] != D or D == 1)):
+                npts, cs_dim, cs_pt = pts.shape[1], pts.shape[1], 1
+            else:
+                npts, cs_dim, cs_pt = pts.shape[0], 1, D
+            if out is None:
+                out = torch.empty((nout, npts), dtype=torch.float64, device=pts.device)
+            st = _stream_handle(stream) if stream is not None else _current_stream()
+            check(lib.intp_eval(self.h, npts, C.cast(pts.data_ptr(), _dp), cs_dim, cs_pt, nout, op,
+                                C.cast(out.data_ptr(), _dp), out.stride(0), out.stride(1), flags, st))
+            return out
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, D))
+        npts = pts.shape[0]
+        res = np.empty((nout, npts))
+        check(lib.intp_eval(self.h, npts, pts.ctypes.data_as(_dp), 1, D, nout, op, res.ctypes.data_as(_dp),
+                            npts, 1, flags | _lib.INTP_EVAL_HOST_MEM, _stream_handle(stream)))
+        return res
+
+    def __call__(self, *x):
+        """operator()(x...) / operator()(coord array): value, no bounds check."""
+        pts = np.asarray(x[0] if len(x) == 1 else x, dtype=np.float64)
+        if pts.ndim <= 1:
+            return float(self.eval(pts.reshape(1, self.dim))[0, 0])
+        return self.eval(pts)[0]
+
+    def at(self, *x):
+        """at(x...): value with the clamped-range check (std::domain_error)."""
+        pts = np.asarray(x[0] if len(x) == 1 else x, dtype=np.float64).reshape(-1, self.dim)
+        v = self.eval(pts, bounds=True)[0]
+        return float(v[0]) if v.size == 1 else v
+
+    def derivative(self, coord, *orders):
+        """derivative(coord, orders...) (Interpolation.hpp:177-205)."""
+        r = np.asarray(orders[0] if len(orders) == 1 else orders, dtype=np.int32).reshape(1, self.dim)
+        pts = np.asarray(coord, dtype=np.float64).reshape(-1, self.dim)
+        v = self.eval(pts, r)[0]
+        return float(v[0]) if v.size == 1 else v
+
+    def derivative_at(self, coord, *orders):
+        """derivative_at(coord, orders...): with the range check."""
+        r = np.asarray(orders[0] if len(orders) == 1 else orders, dtype=np.int32).reshape(1, self.dim)
+        pts = np.asarray(coord, dtype=np.float64).reshape(-1, self.dim)
+        v = self.eval(pts, r, bounds=True)[0]
+        return float(v[0]) if v.size == 1 else v
+
+
+class InterpolationFunctionTemplate:
+    """intp::InterpolationFunctionTemplate<double, D, order>
+    (InterpolationTemplate.hpp:32-581): knots and factored collocation
+    matrices built once, then ``interpolate(values)`` per value mesh."""
+
+    def __init__(self, order: int, shape: Sequence[int], periodic=None, ranges=None, coords=None,
+                 ctx: Optional[Context] = None):
+        lib = _lib.load()
+        self.ctx = ctx or Context.default()
+        self.shape = tuple(int(s) for s in shape)
+        desc, keep = _grid_desc(order, self.shape, periodic, ranges, coords)
+        h = C.c_void_p()
+        check(lib.intp_template_create(self.ctx.h, C.byref(desc), C.byref(h)))
+        del keep
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.intp_template_destroy(self.h)
+            self.h = None
+
+    def interpolate(self, f, stream=None) -> InterpolationFunction:
+        lib = _lib.load()
+        if tuple(f.shape) != self.shape:
+            raise IntpError(_lib.INTP_E_INVALID, "value mesh shape does not match the template")
+        h = C.c_void_p()
+        if _is_torch(f):
+            f = f.contiguous()
+            st = _stream_handle(stream) if stream is not None else _current_stream()
+            check(lib.intp_interpolate(self.h, C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE, C.byref(h), st))
+        else:
+            fa = np.ascontiguousarray(f, dtype=np.float64)
+            check(lib.intp_interpolate(self.h, fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST, C.byref(h),
+                                       _stream_handle(stream)))
+        return InterpolationFunction._wrap(h, self.ctx)

@@ -1,0 +1,119 @@
+"""ctypes binding of libintp_amd.so (include/intp_amd.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libintp_amd.so")
+
+MAX_DIM = 3
+MAX_ORDER = 7
+
+INTP_OK = 0
+INTP_E_INVALID = 1
+INTP_E_DOMAIN = 2
+INTP_E_NOMEM = 3
+INTP_E_HIP = 4
+INTP_E_UNSUPPORTED = 5
+
+INTP_MEM_DEVICE = 0
+INTP_MEM_HOST = 1
+INTP_EVAL_BOUNDS_CHECK = 1
+INTP_EVAL_HOST_MEM = 2
+
+# every symbol include/intp_amd.h declares
+EXPORTS = (
+    "intp_last_error", "intp_status_string", "intp_ctx_create", "intp_ctx_destroy",
+    "intp_template_create", "intp_template_destroy", "intp_interpolate", "intp_build",
+    "intp_spline_from_knots", "intp_spline_retain", "intp_spline_release", "intp_spline_get_info",
+    "intp_spline_download_ctrl", "intp_spline_download_knots", "intp_eval",
+)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("periodic", C.c_int32 * MAX_DIM),
+                ("n", C.c_int64 * MAX_DIM), ("xmin", C.c_double * MAX_DIM), ("xmax", C.c_double * MAX_DIM),
+                ("coords", C.POINTER(C.c_double) * MAX_DIM)]
+
+
+class SplineInfo(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("periodic", C.c_int32 * MAX_DIM),
+                ("uniform", C.c_int32 * MAX_DIM), ("n", C.c_int64 * MAX_DIM), ("nknots", C.c_int64 * MAX_DIM),
+                ("range_lo", C.c_double * MAX_DIM), ("range_hi", C.c_double * MAX_DIM)]
+
+
+_LIB = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads libintp_amd.so; raises if it is missing (build it first with
+    ``python -m bsplineinterpolation_amd.build``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError("libintp_amd.so is not built (%s); run `python -m bsplineinterpolation_amd.build`" % path)
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # SONAME as ROCm's).  Loading torch first makes this library bind to
+    # that copy; loading ours first would leave torch with a second runtime
+    # that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(path)
+    vp, dp, i64p, i32p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+    lib.intp_last_error.restype = C.c_char_p
+    lib.intp_status_string.restype = C.c_char_p
+    lib.intp_status_string.argtypes = [C.c_int]
+    lib.intp_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.intp_ctx_destroy.argtypes = [vp]
+    lib.intp_template_create.argtypes = [vp, C.POINTER(GridDesc), C.POINTER(vp)]
+    lib.intp_template_destroy.argtypes = [vp]
+    lib.intp_interpolate.argtypes = [vp, dp, C.c_int, C.POINTER(vp), vp]
+    lib.intp_build.argtypes = [vp, C.POINTER(GridDesc), dp, C.c_int, C.POINTER(vp), vp]
+    lib.intp_spline_from_knots.argtypes = [vp, C.c_int32, C.c_int32, i32p, i64p, C.POINTER(dp), i64p, dp,
+                                           C.POINTER(vp)]
+    lib.intp_spline_retain.argtypes = [vp]
+    lib.intp_spline_release.argtypes = [vp]
+    lib.intp_spline_get_info.argtypes = [vp, C.POINTER(SplineInfo)]
+    lib.intp_spline_download_ctrl.argtypes = [vp, dp, C.c_int64]
+    lib.intp_spline_download_knots.argtypes = [vp, C.c_int32, dp, C.c_int64]
+    lib.intp_eval.argtypes = [vp, C.c_int64, dp, C.c_int64, C.c_int64, C.c_int32, i32p, dp, C.c_int64,
+                              C.c_int64, C.c_uint32, vp]
+    for name in EXPORTS:
+        getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
+    lib.intp_last_error.restype = C.c_char_p
+    lib.intp_status_string.restype = C.c_char_p
+    _LIB = lib
+    return lib
+
+
+class IntpError(RuntimeError):
+    """A non-zero intp_status; ``code`` is the status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class DomainError(IntpError, ValueError):
+    """INTP_E_DOMAIN: the reference's std::domain_error."""
+
+
+def check(rc: int) -> None:
+    if rc == INTP_OK:
+        return
+    lib = load()
+    msg = lib.intp_last_error().decode(errors="replace")
+    status = lib.intp_status_string(rc).decode()
+    if rc == INTP_E_DOMAIN:
+        raise DomainError(rc, msg)
+    if rc == INTP_E_NOMEM:
+        raise MemoryError(msg)
+    raise IntpError(rc, "%s: %s" % (status, msg))

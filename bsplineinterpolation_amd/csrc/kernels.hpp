@@ -1,0 +1,76 @@
+// kernels.hpp -- launch interface between the C ABI (abi.hip) and the
+// gfx950 kernels (eval.hip, solve.hip).  Plain structs passed by value
+// as kernel arguments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace intp_amd {
+
+constexpr int kMaxDim = 3;
+constexpr int kMaxOrder = 7;
+
+// One dimension of a spline as the evaluation kernels see it.
+struct DimParams {
+    const double* knots;  // nt knots
+    const double* recip;  // nt * order*(order+1)/2 reciprocal knot gaps
+    int64_t n;            // control points along the dimension
+    int64_t stride;       // element stride of the dimension in the table
+    int32_t nt;
+    int32_t periodic;
+    int32_t uniform;
+    int32_t pad_;
+    double lo, hi, period;  // range(d) and its length
+    double inv_dx;          // 1/dx for the uniform span hint
+    double shift;           // hint shift: 1 (periodic) or (order+1)/2
+};
+
+struct EvalParams {
+    DimParams dim[kMaxDim];
+    const double* ctrl;
+    const double* coords;
+    int64_t cs_dim, cs_pt;
+    double* out;
+    int64_t os_out, os_pt;
+    int64_t npts;
+    // gradient kernels: output index written for value (slot 0) and for
+    // d/dx_d (slot d+1), -1 = not requested
+    int32_t slot[kMaxDim + 1];
+    // single-output kernels: the derivative order per dimension and the
+    // output index
+    int32_t order[kMaxDim];
+    int32_t out_index;
+    int* bad;  // set to 1 if a clamped coordinate is outside range (or null)
+};
+
+// Line solves along one axis of a C-order table viewed as
+// (outer, n, inner): line (o, i) holds elements o*n*inner + j*inner + i.
+struct SolveParams {
+    double* x;
+    int64_t n, inner, nlines;
+    const double* lower;   // n * bw
+    const double* upper;   // n * bw
+    const double* diag;    // n
+    const double* bottom;  // bw * n (cyclic)
+    const double* right;   // bw * n (cyclic)
+};
+
+struct ShiftParams {
+    const double* src;
+    double* dst;
+    int64_t total;
+    int32_t dim;
+    int64_t n[kMaxDim];
+    int64_t shift[kMaxDim];
+};
+
+// Returns hipSuccess or the launch error; hipErrorNotSupported when no
+// kernel is compiled for (dim, order).
+hipError_t launch_eval_grad(int dim, int order, const EvalParams& p, hipStream_t s);
+hipError_t launch_eval_single(int dim, int order, const EvalParams& p, hipStream_t s);
+hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s);
+hipError_t launch_shift_copy(const ShiftParams& p, hipStream_t s);
+
+}  // namespace intp_amd

@@ -1,0 +1,159 @@
+/*
+ * intp_amd.h -- C ABI of the MI355X (gfx950) B-spline interpolation library
+ * libintp_amd.so.  Plain pointers and sizes; no C++ or torch types.
+ *
+ * The reference (12ff54e/BSplineInterpolation) is a header-only C++ template
+ * library with no C ABI of its own; each entry point below names the
+ * reference member it replaces.  The C++ drop-in header
+ * include/intp/Interpolation.hpp rebuilds the reference's class surface on
+ * top of these calls.
+ *
+ * Conventions (the reference's test configuration, test/CMakeLists.txt:46-47):
+ *  - periodic dimensions store n values with spacing (xmax - xmin)/n
+ *    (INTP_PERIODIC_NO_DUMMY_POINT);
+ *  - value meshes are C-order (last index fastest), Mesh.hpp:87-95;
+ *  - every call returns an intp_status; intp_last_error() describes the
+ *    last failure on the calling thread.
+ *
+ * Thread safety: splines and templates are immutable after creation, so
+ * any number of threads may evaluate one spline concurrently, on any
+ * streams (the reference's evaluation members are const and reentrant,
+ * Interpolation.hpp:132-244).  Creation and release are thread-safe.
+ */
+#ifndef INTP_AMD_H
+#define INTP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define INTP_MAX_DIM 3   /* dimensions with compiled kernels            */
+#define INTP_MAX_ORDER 7 /* spline orders with compiled kernels         */
+
+typedef enum {
+    INTP_OK = 0,
+    INTP_E_INVALID = 1,     /* bad argument; reference: std::runtime_error
+                               from INTP_ASSERT (util.hpp:247-258)        */
+    INTP_E_DOMAIN = 2,      /* coordinate out of range; reference:
+                               std::domain_error (Interpolation.hpp:482-491) */
+    INTP_E_NOMEM = 3,       /* device allocation failed; std::bad_alloc  */
+    INTP_E_HIP = 4,         /* HIP runtime error                         */
+    INTP_E_UNSUPPORTED = 5  /* dimension/order without compiled kernels  */
+} intp_status;
+
+/* where a caller's buffer lives */
+#define INTP_MEM_DEVICE 0
+#define INTP_MEM_HOST 1
+
+/* eval flags */
+#define INTP_EVAL_BOUNDS_CHECK 1u /* at() / derivative_at() semantics     */
+#define INTP_EVAL_HOST_MEM 2u     /* coords and outputs are host memory   */
+
+typedef struct intp_ctx_s* intp_ctx;
+typedef struct intp_template_s* intp_template;
+typedef struct intp_spline_s* intp_spline;
+
+/* Grid description: one InterpolationFunctionTemplate's arguments
+ * (InterpolationTemplate.hpp:60-79; Interpolation.hpp:92-107).
+ * coords[d] == NULL: uniform dimension on [xmin[d], xmax[d]]
+ *   (the std::pair<T,T>(min, max) range, Interpolation.hpp:322-362);
+ * coords[d] != NULL: host array of n[d] + periodic[d] node coordinates
+ *   (the iterator-pair range, Interpolation.hpp:365-464).               */
+typedef struct {
+    int32_t dim;
+    int32_t order;
+    int32_t periodic[INTP_MAX_DIM];
+    int64_t n[INTP_MAX_DIM];
+    double xmin[INTP_MAX_DIM];
+    double xmax[INTP_MAX_DIM];
+    const double* coords[INTP_MAX_DIM];
+} intp_grid_desc;
+
+/* Properties of a spline: InterpolationFunction::periodicity / uniform /
+ * range / get_order (Interpolation.hpp:248-267).                         */
+typedef struct {
+    int32_t dim;
+    int32_t order;
+    int32_t periodic[INTP_MAX_DIM];
+    int32_t uniform[INTP_MAX_DIM];
+    int64_t n[INTP_MAX_DIM];  /* control points per dimension */
+    int64_t nknots[INTP_MAX_DIM];
+    double range_lo[INTP_MAX_DIM];
+    double range_hi[INTP_MAX_DIM];
+} intp_spline_info;
+
+/* ---- errors -------------------------------------------------------- */
+const char* intp_last_error(void);
+const char* intp_status_string(int status);
+
+/* ---- context ------------------------------------------------------- */
+/* Binds a device (hipSetDevice) for every later call made with objects of
+ * this context.  No reference counterpart (the reference has no device). */
+int intp_ctx_create(int device, intp_ctx* out);
+int intp_ctx_destroy(intp_ctx ctx);
+
+/* ---- template: knots + factored collocation matrices ---------------- */
+/* Replaces InterpolationFunctionTemplate<T,D,O,U>(periodicity,
+ * MeshDimension, x_ranges...) (InterpolationTemplate.hpp:60-79): builds
+ * knot vectors (Interpolation.hpp:322-480), assembles and LU-factors the
+ * per-dimension collocation matrices (InterpolationTemplate.hpp:254-446,
+ * BandLU.hpp:103-213) and uploads them.                                  */
+int intp_template_create(intp_ctx ctx, const intp_grid_desc* desc, intp_template* out);
+int intp_template_destroy(intp_template t);
+
+/* Replaces InterpolationFunctionTemplate::interpolate(mesh)
+ * (InterpolationTemplate.hpp:118-143, 448-580): solves for the control
+ * points of `values` (C-order, n[0] x ... x n[dim-1] doubles) on the GPU
+ * and returns a new spline holding them in HBM.  values_mem is
+ * INTP_MEM_HOST or INTP_MEM_DEVICE.  Asynchronous on `stream`
+ * (hipStream_t, NULL = default) for device input.                       */
+int intp_interpolate(intp_template t, const double* values, int values_mem, intp_spline* out,
+                     void* stream);
+
+/* Replaces InterpolationFunction<T,D,O,U>(periodicity, f_mesh,
+ * x_ranges...) (Interpolation.hpp:92-107): template + interpolate.      */
+int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem,
+               intp_spline* out, void* stream);
+
+/* Replaces BSpline<T,D,O,U>(periodicity, ctrl_pts, knot_pairs...)
+ * (BSpline.hpp:188-210): a spline from explicit knot vectors (host arrays,
+ * nknots[d] each) and control points (host, C-order, n[d] each with
+ * nknots[d] = n[d] + order + 1 (+order if periodic)).                    */
+int intp_spline_from_knots(intp_ctx ctx, int32_t dim, int32_t order, const int32_t* periodic,
+                           const int64_t* n, const double* const* knots, const int64_t* nknots,
+                           const double* ctrl, intp_spline* out);
+
+/* Reference counting (value semantics of the reference: a copy of a built
+ * spline shares its immutable device table).                            */
+int intp_spline_retain(intp_spline s);
+int intp_spline_release(intp_spline s);
+int intp_spline_get_info(intp_spline s, intp_spline_info* out);
+/* Copies the control points (C-order) to host memory; synchronous.       */
+int intp_spline_download_ctrl(intp_spline s, double* host, int64_t count);
+/* Copies knot vector `d` to host memory (nknots[d] doubles).             */
+int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t count);
+
+/* ---- evaluation ---------------------------------------------------- */
+/* Replaces operator() / derivative() and, with INTP_EVAL_BOUNDS_CHECK,
+ * at() / derivative_at() (Interpolation.hpp:132-244; BSpline.hpp:305-550),
+ * batched over points:
+ *   coordinate d of point i : coords[d * cs_dim + i * cs_pt]
+ *   output o of point i     : out[o * os_out + i * os_pt]
+ *   derivative orders       : orders[o * dim + d], o < nout
+ * (SoA: cs_dim = npts, cs_pt = 1; AoS: cs_dim = 1, cs_pt = dim.)
+ * All outputs of one call share one stencil gather.  Out-of-range
+ * coordinates of clamped dimensions extrapolate the boundary polynomial,
+ * periodic ones wrap; with INTP_EVAL_BOUNDS_CHECK the call returns
+ * INTP_E_DOMAIN (and synchronizes `stream`) if any clamped coordinate is
+ * outside range(d).  Orders above the spline order give 0.              */
+int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
+              int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
+              uint32_t flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

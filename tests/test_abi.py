@@ -1,0 +1,65 @@
+"""CPU checks of the product library: it builds, loads, and exports every
+entry point include/intp_amd.h declares; host-side argument checks."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "intp_amd.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(intp_\w+)\s*\(", txt, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from bsplineinterpolation_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from bsplineinterpolation_amd import build
+        build.build()
+    return _lib.load()
+
+
+def test_header_declares_expected_entry_points():
+    from bsplineinterpolation_amd import _lib
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    from bsplineinterpolation_amd import _lib
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (intp_\w+)", out))
+    for name in _declared():
+        assert name in exported, name
+        assert hasattr(lib, name)
+
+
+def test_status_strings(lib):
+    for code in range(6):
+        assert lib.intp_status_string(code)
+    assert lib.intp_last_error() is not None
+
+
+def test_null_arguments_are_rejected_without_a_device(lib):
+    from bsplineinterpolation_amd import _lib
+    h = C.c_void_p()
+    assert lib.intp_template_create(None, None, C.byref(h)) == _lib.INTP_E_INVALID
+    assert b"null" in lib.intp_last_error()
+    assert lib.intp_eval(None, 1, None, 1, 1, 1, None, None, 1, 1, 0, None) == _lib.INTP_E_INVALID
+    assert lib.intp_spline_release(None) == _lib.INTP_OK
+
+
+def test_product_does_not_import_the_oracle():
+    """The product path has no route into the oracle (test infrastructure)."""
+    pkg = os.path.join(ROOT, "bsplineinterpolation_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.replace("the oracle", ""), f
+                assert "liborc" not in txt, f
