@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Benchmark: 3D 256^3 order-3 FP64 build + 2^26 evaluations on one MI355X.
+
+Workload (the reference configuration the metric is quoted on): values
+sin(x) exp(-(y-1/2)^2/0.05) cos(4 pi z) + N(0, 1e-3^2) on a 256^3 mesh over
+[0,2pi) x [0,1] x [0,1], dim 0 periodic, dims 1-2 clamped; 2^26 uniform
+query points; outputs value, d/dx, d/dy, d/dz.  Seeded synthetic data
+(bsplineinterpolation_amd/datagen.py).
+
+One step = the full constructor (knots, collocation LU, control-point
+solve; intp_build) + one evaluation call over all 2^26 points, with the
+values and query points already resident in HBM.  ``value`` = query points
+per second over the whole step.
+
+The eval kernel dominates the step; its roofline entry uses the
+algorithmic bytes 2^26*(3*8 + 4*8) + 256^3*8 (coordinates in, outputs out,
+table read once) over its HIP-event-timed average duration on the stream it
+runs on.  The CPU baseline times the compiled reference
+(oracle/_ref/libintp_ref.so; the oracle restatement if that is absent):
+the constructor at full size with the reference's 8 pool workers, and
+evaluation of a sample of the same queries with one thread per host core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _cores() -> int:
+    n = os.cpu_count() or 1
+    for var in ("OMP_NUM_THREADS",):
+        v = os.environ.get(var)
+        if v and v.isdigit():
+            n = min(n, int(v))
+    try:
+        n = min(n, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    return max(1, n)
+
+
+def rel_stats(got, want):
+    d = got - want
+    return dict(norm=float(math.sqrt(np.sum(d * d) / np.sum(want * want))),
+                max=float(np.max(np.abs(d)) / np.max(np.abs(want))),
+                rms=float(math.sqrt(np.mean(d * d))))
+
+
+def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
+    """Reference CPU path, bounded sample.  Returns (dict, error stats)."""
+    import oracle
+    from oracle import oracle as omod
+    kind = "reference" if omod.ref_available() else "port"
+    Cls = oracle.RefSpline if kind == "reference" else oracle.OracleSpline
+    kw = {} if kind == "reference" else {"nthreads": 8}
+    t0 = time.perf_counter()
+    s = Cls(3, w["f"], periodic=w["periodic"], ranges=w["ranges"], **kw)
+    t_build = time.perf_counter() - t0
+    q = np.ascontiguousarray(w["queries"][:, :sample_mt].T)
+    cores = _cores()
+    t0 = time.perf_counter()
+    ref = s.eval(q, w["orders"], nthreads=cores)
+    t_mt = (time.perf_counter() - t0) / sample_mt
+    t0 = time.perf_counter()
+    s.eval(q[:sample_st], w["orders"], nthreads=1)
+    t_st = (time.perf_counter() - t0) / sample_st
+    nq = w["nq"]
+    step_s = t_build + t_mt * nq
+    errs = [rel_stats(gpu_out_sample[i], ref[i]) for i in range(len(w["orders"]))]
+    return dict(value=nq / step_s, unit="points/s", cores=cores, kind=kind,
+                sample=("constructor at full size (%.3f s, 8 pool workers + caller); evaluation of value + 3 "
+                        "partials on %d of the 2^26 queries with %d threads (%.3f us/point), scaled to 2^26; "
+                        "single-thread %.3f us/point on %d points" % (t_build, sample_mt, cores, t_mt * 1e6,
+                                                                       t_st * 1e6, sample_st)),
+                build_s=t_build, eval_us_per_point_mt=t_mt * 1e6, eval_us_per_point_st=t_st * 1e6,
+                step_s=step_s), errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--queries", type=int, default=1 << 26)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU layout is described in DESIGN.md, "
+              "--gpus %d is not run" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import bsplineinterpolation_amd as amd
+    from bsplineinterpolation_amd import datagen
+
+    n, nq = args.n, args.queries
+    print("bench: generating inputs (%d^3 mesh, %d queries)" % (n, nq), file=sys.stderr, flush=True)
+    w = datagen.workload_3d_256_cubic(n=n, nq=nq)
+    w["nq"] = nq
+    dev = torch.device("cuda:0")
+    f_d = torch.from_numpy(w["f"]).to(dev)
+    q_d = torch.from_numpy(w["queries"]).to(dev)  # SoA [3, nq]
+    out_d = torch.empty((4, nq), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx = amd.Context(0)
+    per, rg, orders = w["periodic"], w["ranges"], w["orders"]
+    stream = torch.cuda.current_stream()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        s = amd.InterpolationFunction(3, f_d, periodic=per, ranges=rg, ctx=ctx)
+        if ev is not None:
+            ev[1].record(stream)
+        s.eval(q_d, orders, out=out_d)
+        if ev is not None:
+            ev[2].record(stream)
+        return s
+
+    print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    keep = None
+    for _ in range(args.warmup):
+        keep = step()
+    torch.cuda.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        keep = step(events[i])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    build_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
+    eval_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+
+    # control-point solve alone: template built once, interpolate timed
+    tmpl = amd.InterpolationFunctionTemplate(3, (n, n, n), periodic=per, ranges=rg, ctx=ctx)
+    tmpl.interpolate(f_d)
+    torch.cuda.synchronize()
+    sev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(5)]
+    for e in sev:
+        e[0].record(stream)
+        sp = tmpl.interpolate(f_d)
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    solve_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in sev]))
+    del sp
+
+    N = n ** 3
+    b_eval = nq * (3 * 8 + 4 * 8) + N * 8
+    b_solve = 2 * 3 * N * 8
+    eval_gbs = b_eval / (eval_ms * 1e-3) / 1e9
+    solve_gbs = b_solve / (solve_ms * 1e-3) / 1e9
+    value = nq / (ms_per_step * 1e-3)
+
+    result = {
+        "metric": "3D 256^3 cubic: evals/s + solve GB/s vs HBM peak; served p50/p99 latency",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64/Box-Muller generator, bsplineinterpolation_amd/datagen.py)",
+        "config": {"workload": "3D %d^3 order-3 FP64 build (dim 0 periodic, dims 1-2 clamped) + %d queries x "
+                               "(value, d/dx, d/dy, d/dz)" % (n, nq),
+                   "mesh": [n, n, n], "order": 3, "periodic": per, "queries": nq, "outputs": 4},
+        "breakdown_ms": {"build_full_constructor": build_ms, "solve_only": solve_ms, "eval": eval_ms},
+        "solve": {"gbs": solve_gbs, "frac_hbm": solve_gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_solve},
+        "roofline": {"bound": "hbm", "kernel": "eval_grad_kernel<3,3>", "achieved": eval_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": eval_gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes": b_eval, "traffic": None},
+    }
+
+    if not args.no_cpu:
+        m = min(args.cpu_sample, nq)
+        print("bench: CPU baseline (sample %d points)" % m, file=sys.stderr, flush=True)
+        sample = out_d[:, :m].cpu().numpy()
+        cb, errs = cpu_baseline(w, sample, m, min(1 << 15, m))
+        result["cpu_baseline"] = cb
+        result["speedup_vs_cpu"] = value / cb["value"]
+        result["error_vs_cpu_reference"] = {name: e for name, e in zip(["value", "d/dx", "d/dy", "d/dz"], errs)}
+    del keep
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

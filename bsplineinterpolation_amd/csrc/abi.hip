@@ -67,13 +67,15 @@ struct GridData {
     DimSetup setup[kMaxDim];
     DevArray knots[kMaxDim], recip[kMaxDim];
     DevArray lower[kMaxDim], upper[kMaxDim], diag[kMaxDim], bottom[kMaxDim], right[kMaxDim];
+    DevArray poly[kMaxDim];
     bool has_factors = false;
     int64_t total = 1;
 
     ~GridData() {
         DeviceScope scope(device);
         for (int d = 0; d < kMaxDim; ++d)
-            for (DevArray* a : {&knots[d], &recip[d], &lower[d], &upper[d], &diag[d], &bottom[d], &right[d]})
+            for (DevArray* a : {&knots[d], &recip[d], &lower[d], &upper[d], &diag[d], &bottom[d], &right[d],
+                                &poly[d]})
                 if (a->p) (void)hipFree(a->p);
     }
 };
@@ -92,6 +94,7 @@ int upload_grid(GridData& g) {
         int rc;
         if ((rc = upload(s.knots, g.knots[d])) != INTP_OK) return rc;
         if ((rc = upload(s.recip, g.recip[d])) != INTP_OK) return rc;
+        if ((rc = upload(s.poly, g.poly[d])) != INTP_OK) return rc;
         if (g.has_factors) {
             if ((rc = upload(s.lu.lower, g.lower[d])) != INTP_OK) return rc;
             if ((rc = upload(s.lu.upper, g.upper[d])) != INTP_OK) return rc;
@@ -220,7 +223,24 @@ DimParams dim_params(const GridData& g, int d) {
     p.period = s.hi - s.lo;
     p.inv_dx = s.uniform ? 1.0 / s.dx : 0.0;
     p.shift = s.periodic ? 1.0 : 0.5 * static_cast<double>(g.order + 1);
+    p.xmin = s.xmin;
+    p.xmax = s.xmax;
+    p.dx = s.dx;
+    p.half = s.half;
+    p.poly = g.poly[d].p;
+    p.nleft = s.nleft;
+    p.nright = s.nright;
+    p.nclass = s.nclass;
     return p;
+}
+
+// The uniform fast path needs every dimension uniform (knots in closed
+// form, basis polynomial classes uploaded) and 32-bit table offsets.
+bool uniform_path(const GridData& g) {
+    if (g.total >= (int64_t{1} << 32)) return false;
+    for (int d = 0; d < g.dim; ++d)
+        if (!g.setup[d].uniform || g.poly[d].p == nullptr) return false;
+    return true;
 }
 
 }  // namespace
@@ -473,11 +493,12 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         }
     }
     hipError_t e = hipSuccess;
-    if (any_grad) e = launch_eval_grad(D, g.order, p, stream);
+    const bool fast = uniform_path(g);
+    if (any_grad) e = launch_eval_grad(D, g.order, fast, p, stream);
     for (size_t i = 0; e == hipSuccess && i < single.size(); ++i) {
         for (int d = 0; d < D; ++d) p.order[d] = orders[single[i] * D + d];
         p.out_index = single[i];
-        e = launch_eval_single(D, g.order, p, stream);
+        e = launch_eval_single(D, g.order, fast, p, stream);
     }
     if (e != hipSuccess) {
         cleanup();

@@ -25,6 +25,10 @@ struct DimParams {
     double lo, hi, period;  // range(d) and its length
     double inv_dx;          // 1/dx for the uniform span hint
     double shift;           // hint shift: 1 (periodic) or (order+1)/2
+    // uniform dimensions: knot formula and basis-polynomial classes
+    double xmin, xmax, dx, half;
+    const double* poly;  // nclass * (order+1)^2 coefficients
+    int32_t nleft, nright, nclass, pad2_;
 };
 
 struct EvalParams {
@@ -67,9 +71,10 @@ struct ShiftParams {
 };
 
 // Returns hipSuccess or the launch error; hipErrorNotSupported when no
-// kernel is compiled for (dim, order).
-hipError_t launch_eval_grad(int dim, int order, const EvalParams& p, hipStream_t s);
-hipError_t launch_eval_single(int dim, int order, const EvalParams& p, hipStream_t s);
+// kernel is compiled for (dim, order).  `uniform`: every dimension is
+// uniform and the table has fewer than 2^32 elements (fast path).
+hipError_t launch_eval_grad(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t launch_eval_single(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s);
 hipError_t launch_shift_copy(const ShiftParams& p, hipStream_t s);
 

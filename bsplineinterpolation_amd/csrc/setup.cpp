@@ -122,6 +122,71 @@ void fill_recip(int order, DimSetup& s) {
             }
 }
 
+// Power-basis coefficients (in u = (x - t[mu]) / dx) of the order+1 B-splines
+// non-zero on span mu: the Cox-de Boor recursion run on polynomials.
+void span_polynomials(const std::vector<double>& t, int k, int64_t mu, double dx, double* out) {
+    const int w = k + 1;
+    std::vector<double> tau(static_cast<size_t>(2 * k + 2), 0.0);  // tau[i + k] = knot mu+i, i in [-k, k+1]
+    for (int i = -k; i <= k + 1; ++i) {
+        const int64_t idx = mu + i;
+        if (idx >= 0 && idx < static_cast<int64_t>(t.size()))
+            tau[static_cast<size_t>(i + k)] = (t[static_cast<size_t>(idx)] - t[static_cast<size_t>(mu)]) / dx;
+    }
+    auto T = [&](int i) { return tau[static_cast<size_t>(i + k)]; };
+    std::vector<double> b(static_cast<size_t>(w * w), 0.0);  // b[j*w + p]
+    b[static_cast<size_t>(k * w)] = 1.0;
+    for (int i = 1; i <= k; ++i) {
+        const int first = k - i;
+        for (int j = 0; j <= i; ++j) {
+            const int idx = first + j;
+            std::vector<double> nb(static_cast<size_t>(w), 0.0);
+            if (j != 0) {  // b[idx] * (u - tau_{j-i}) / (tau_j - tau_{j-i})
+                const double lo = T(j - i), den = T(j) - T(j - i);
+                for (int p = 0; p < w; ++p) {
+                    const double c = b[static_cast<size_t>(idx * w + p)] / den;
+                    if (p + 1 < w) nb[static_cast<size_t>(p + 1)] += c;
+                    nb[static_cast<size_t>(p)] -= c * lo;
+                }
+            }
+            if (idx != k) {  // b[idx+1] * (tau_{j+1} - u) / (tau_{j+1} - tau_{j+1-i})
+                const double hi = T(j + 1), den = T(j + 1) - T(j + 1 - i);
+                for (int p = 0; p < w; ++p) {
+                    const double c = b[static_cast<size_t>((idx + 1) * w + p)] / den;
+                    nb[static_cast<size_t>(p)] += c * hi;
+                    if (p + 1 < w) nb[static_cast<size_t>(p + 1)] -= c;
+                }
+            }
+            for (int p = 0; p < w; ++p) b[static_cast<size_t>(idx * w + p)] = nb[static_cast<size_t>(p)];
+        }
+    }
+    for (int q = 0; q < w * w; ++q) out[q] = b[static_cast<size_t>(q)];
+}
+
+void fill_uniform_classes(int k, DimSetup& s, double xmin, double xmax) {
+    const int64_t nt = static_cast<int64_t>(s.knots.size());
+    const int64_t last = nt - k - 2;
+    const int64_t nspan = last - k + 1;
+    s.xmin = xmin;
+    s.xmax = xmax;
+    s.half = 0.5 * static_cast<double>(nt - (s.n + (s.periodic ? 1 : 0)));
+    if (s.periodic) {
+        s.nleft = s.nright = 0;
+    } else {
+        // span mu is interior when knots mu-k+1 .. mu+k all lie in [k+1, nt-k-2]
+        s.nleft = static_cast<int>(std::min<int64_t>(nspan, k));
+        s.nright = static_cast<int>(std::min<int64_t>(nspan - s.nleft, k));
+    }
+    s.nclass = 1 + s.nleft + s.nright;
+    const int w = k + 1;
+    s.poly.assign(static_cast<size_t>(s.nclass * w * w), 0.0);
+    const int64_t interior = s.periodic ? k : k + s.nleft;
+    if (interior <= last - s.nright) span_polynomials(s.knots, k, interior, s.dx, &s.poly[0]);
+    for (int c = 0; c < s.nleft; ++c)
+        span_polynomials(s.knots, k, k + c, s.dx, &s.poly[static_cast<size_t>((1 + c) * w * w)]);
+    for (int c = 0; c < s.nright; ++c)
+        span_polynomials(s.knots, k, last - c, s.dx, &s.poly[static_cast<size_t>((1 + s.nleft + c) * w * w)]);
+}
+
 void set_range(int order, DimSetup& s) {
     const size_t nt = s.knots.size();
     s.lo = s.knots[static_cast<size_t>(order)];
@@ -175,6 +240,8 @@ std::string setup_dimension(int order, bool periodic, int64_t n, double xmin, do
             for (int64_t i = k + 1; i < nt - k - 1; ++i) t[i] = xmin + (static_cast<double>(i) - half) * s.dx;
             for (int64_t i = nt - k - 1; i < nt; ++i) t[i] = xmax;
         }
+        set_range(k, s);
+        fill_uniform_classes(k, s, xmin, xmax);
     } else {
         // node-coordinate knots: Interpolation.hpp:365-447
         s.uniform = false;

@@ -37,6 +37,19 @@ struct DimSetup {
     // recip[mu*T + i*(i-1)/2 + a-1] = 1 / (t[mu+a] - t[mu+a-i]),
     // i = 1..order, a = 1..i, T = order*(order+1)/2
     std::vector<double> recip;
+
+    // Uniform dimensions only: the knot formula and the basis polynomials.
+    // t(i) = xmin for i <= order (clamped), xmax for i >= nt-order-1
+    // (clamped), else xmin + (i - half) * dx.
+    double xmin = 0, xmax = 0, half = 0;
+    // Spans whose 2*order knots all follow the formula share one set of
+    // basis polynomials (class 0).  The first `nleft` spans (mu = order..)
+    // and the last `nright` spans of a clamped dimension get their own
+    // classes: 1 + (mu - order) and 1 + nleft + (last - mu).
+    int nleft = 0, nright = 0, nclass = 1;
+    // poly[(c*(order+1) + j)*(order+1) + p]: coefficient of u^p of the j-th
+    // non-zero B-spline on a span of class c, u = (x - t(mu)) / dx.
+    std::vector<double> poly;
 };
 
 // Returns an empty string on success, else the reason.

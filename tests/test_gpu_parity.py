@@ -162,3 +162,24 @@ def test_gpu_2d_64_cubic_workload(amd, orc):
     for i in range(2):
         assert rel_err(got[i], want[i]) < TOL
         assert np.max(np.abs(got[i] - want[i])) <= 1e-12 * np.max(np.abs(want[i]))
+
+
+@pytest.mark.parametrize("k,per", [(2, True), (3, True), (4, False), (5, False), (3, False), (6, True)])
+def test_gpu_highest_derivative_on_every_knot(amd, orc, k, per):
+    """The order-k derivative is piecewise constant, so evaluating it exactly
+    on a knot shows which span was chosen: it must be the reference's
+    (t[mu] <= x < t[mu+1]).  Ranges are offset from 0 so that knots are
+    not exact multiples of dx (an FMA in the knot formula would move them)."""
+    rng = np.random.default_rng(40 + k)
+    n = 23
+    f = rng.standard_normal(n)
+    rg = [(-0.4620982391885941, 0.7015415866960515)]
+    o = orc.OracleSpline(k, f, periodic=[per], ranges=rg)
+    g = amd.InterpolationFunction(k, f, periodic=[per], ranges=rg)
+    t = o.knots(0)
+    lo, hi = o.range(0)
+    pts = t[(t >= lo) & (t <= hi)][:, None]
+    for r in (k, k - 1, 0):
+        want = o.eval(pts, [[r]])[0]
+        got = g.eval(pts, [[r]])[0]
+        assert rel_err(got, want) < TOL, r
