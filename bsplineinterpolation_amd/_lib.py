@@ -25,6 +25,7 @@ INTP_MEM_DEVICE = 0
 INTP_MEM_HOST = 1
 INTP_EVAL_BOUNDS_CHECK = 1
 INTP_EVAL_HOST_MEM = 2
+INTP_EVAL_UNGROUPED = 4
 
 # every symbol include/intp_amd.h declares
 EXPORTS = (

@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libintp_amd.so")
 OBJ = os.path.join(PKG, "_build")
 ARCH = os.environ.get("INTP_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["abi.hip", "eval.hip", "solve.hip"]
+HIP_SOURCES = ["abi.hip", "bin.hip", "eval.hip", "solve.hip"]
 CXX_SOURCES = ["setup.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall",

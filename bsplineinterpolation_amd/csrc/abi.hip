@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -232,6 +233,45 @@ DimParams dim_params(const GridData& g, int d) {
     p.nright = s.nright;
     p.nclass = s.nclass;
     return p;
+}
+
+// Batches at least this large are grouped by brick before evaluation.
+constexpr int64_t kGroupMinPoints = int64_t{1} << 20;
+
+// Brick of B^D control points per bin, B a power of two: the largest B
+// whose (B + order)^D patch stays within 16 KB (well inside a CU's L1),
+// coarsened until the brick key fits 16 bits.
+BinParams bin_params(const GridData& g) {
+    BinParams bp{};
+    bp.dim_count = g.dim;
+    int B = 1;
+    auto patch = [&](int b) {
+        double v = 8.0;
+        for (int d = 0; d < g.dim; ++d) v *= static_cast<double>(b + g.order);
+        return v;
+    };
+    while (patch(2 * B) <= 16384.0) B *= 2;
+    for (;;) {
+        int64_t nbin = 1;
+        for (int d = 0; d < g.dim; ++d) nbin *= (g.setup[d].n + B - 1) / B;
+        if (nbin <= 65536) break;
+        B *= 2;
+    }
+    for (int d = 0; d < g.dim; ++d) {
+        const DimSetup& s = g.setup[d];
+        BinDim& b = bp.dim[d];
+        b.lo = s.lo;
+        b.period = s.hi - s.lo;
+        b.inv_period = 1.0 / b.period;
+        b.inv_dx = 1.0 / s.dx;
+        b.periodic = s.periodic ? 1 : 0;
+        b.ncell = static_cast<int32_t>(s.n);
+        int shift = 0;
+        while ((1 << shift) < B) ++shift;
+        b.brick_shift = shift;
+        b.nbrick = static_cast<int32_t>((s.n + B - 1) / B);
+    }
+    return bp;
 }
 
 // The uniform fast path needs every dimension uniform (knots in closed
@@ -494,12 +534,50 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     }
     hipError_t e = hipSuccess;
     const bool fast = uniform_path(g);
+    BinRecord* rec_a = nullptr;
+    BinRecord* rec_b = nullptr;
+    uint32_t* bin_scratch = nullptr;
+    auto free_bins = [&]() {
+        if (rec_a) (void)hipFreeAsync(rec_a, stream);
+        if (rec_b) (void)hipFreeAsync(rec_b, stream);
+        if (bin_scratch) (void)hipFreeAsync(bin_scratch, stream);
+    };
+    if (fast && D >= 2 && npts >= kGroupMinPoints && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_UNGROUPED) &&
+        bin_scan_tiles(npts) <= 4096) {
+        BinParams bp = bin_params(g);
+        bp.coords = p.coords;
+        bp.cs_dim = p.cs_dim;
+        bp.cs_pt = p.cs_pt;
+        bp.npts = npts;
+        const int64_t nh = bin_hist_entries(npts), nt = bin_scan_tiles(npts);
+        e = hipMallocAsync(reinterpret_cast<void**>(&rec_a), static_cast<size_t>(npts) * sizeof(BinRecord), stream);
+        if (e == hipSuccess)
+            e = hipMallocAsync(reinterpret_cast<void**>(&rec_b), static_cast<size_t>(npts) * sizeof(BinRecord),
+                               stream);
+        if (e == hipSuccess)
+            e = hipMallocAsync(reinterpret_cast<void**>(&bin_scratch), static_cast<size_t>(2 * nh + nt + npts) * 4,
+                               stream);
+        if (e == hipSuccess)
+            e = launch_binning(bp, bin_scratch, bin_scratch + nh, bin_scratch + 2 * nh, bin_scratch + 2 * nh + nt,
+                               rec_a, rec_b, stream);
+        if (e != hipSuccess) {
+            free_bins();
+            cleanup();
+            return hip_fail(e, "grouping query points");
+        }
+        p.coords = reinterpret_cast<const double*>(rec_b);
+        p.cs_dim = 1;
+        p.cs_pt = sizeof(BinRecord) / sizeof(double);
+        p.perm = reinterpret_cast<const uint32_t*>(rec_b) + offsetof(BinRecord, idx) / sizeof(uint32_t);
+        p.perm_stride = sizeof(BinRecord) / sizeof(uint32_t);
+    }
     if (any_grad) e = launch_eval_grad(D, g.order, fast, p, stream);
     for (size_t i = 0; e == hipSuccess && i < single.size(); ++i) {
         for (int d = 0; d < D; ++d) p.order[d] = orders[single[i] * D + d];
         p.out_index = single[i];
         e = launch_eval_single(D, g.order, fast, p, stream);
     }
+    free_bins();
     if (e != hipSuccess) {
         cleanup();
         (void)hipStreamSynchronize(stream);

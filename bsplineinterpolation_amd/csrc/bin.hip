@@ -1,0 +1,247 @@
+// bin.hip -- groups query points by brick of the control-point grid before
+// evaluation, so that the points one workgroup evaluates share a
+// cache-resident patch of the table instead of gathering ~20 scattered
+// cache lines each from the whole table.
+//
+// Two counting-sort passes over a 16-bit brick key (low byte, then high
+// byte).  Each pass: per-block LDS histograms (no global atomics), a
+// device-wide exclusive scan over the digit-major histogram, and a scatter
+// with LDS-atomic ranks.  Ranks inside a block are not stable, which only
+// blurs the grouping within one 2048-point block; the evaluation result
+// does not depend on the order.  Records carry the original coordinates
+// and index, so the evaluation reads them sequentially and writes each
+// result to its original position.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace intp_amd {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 8;
+constexpr int kChunk = kThreads * kPerThread;  // points per block
+constexpr int kRadix = 256;
+constexpr int kScanTile = 4096;  // entries per scan block
+
+__device__ __forceinline__ uint32_t brick_key(const BinParams& p, const double* x) {
+    uint32_t key = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) {
+        if (d >= p.dim_count) break;
+        const BinDim& b = p.dim[d];
+        double r = x[d] - b.lo;
+        if (b.periodic && !(r >= 0.0 && r < b.period)) r -= floor(r * b.inv_period) * b.period;
+        double c = floor(r * b.inv_dx);
+        c = fmin(fmax(c, 0.0), static_cast<double>(b.ncell - 1));  // NaN -> 0
+        key = key * static_cast<uint32_t>(b.nbrick) + (static_cast<uint32_t>(c) >> b.brick_shift);
+    }
+    return key;
+}
+
+__device__ __forceinline__ void load_point(const BinParams& p, int64_t i, double* x) {
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
+}
+
+// ---- exclusive scan of n uint32 (n <= kScanTile * kScanTile) ------------
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wave_sum[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wave) before += wave_sum[w];
+        all += wave_sum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+// pass == 0: keys from caller coordinates, digit = low byte
+// pass == 1: keys from records, digit = high byte
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kChunk;
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t i = first + j * kThreads + threadIdx.x;
+        if (i >= p.npts) break;
+        uint32_t digit;
+        if constexpr (PASS == 0) {
+            double x[kMaxDim];
+            load_point(p, i, x);
+            digit = brick_key(p, x) & 0xFFu;
+        } else {
+            digit = (p.key_in[i] >> 8) & 0xFFu;
+        }
+        atomicAdd(&h[digit], 1u);
+    }
+    __syncthreads();
+    hist[static_cast<int64_t>(threadIdx.x) * p.nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Scatter of one chunk: records are first placed in LDS in digit order
+// (local rank from an LDS atomic, local start from a block scan of the
+// chunk's digit counts), then written out from LDS so that consecutive
+// lanes write consecutive slots of one digit's run.
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, const uint32_t* __restrict__ offs) {
+    __shared__ uint32_t cnt[kRadix];
+    __shared__ uint32_t gbase[kRadix];
+    __shared__ uint32_t lstart[kRadix];
+    __shared__ BinRecord stage[kChunk];
+    cnt[threadIdx.x] = 0;
+    gbase[threadIdx.x] = offs[static_cast<int64_t>(threadIdx.x) * p.nblocks + blockIdx.x];
+    __syncthreads();
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kChunk;
+    BinRecord r[kPerThread];
+    uint32_t digit[kPerThread], rank[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t i = first + j * kThreads + threadIdx.x;
+        digit[j] = 0;
+        rank[j] = 0;
+        if (i < p.npts) {
+            if constexpr (PASS == 0) {
+                load_point(p, i, r[j].x);
+                r[j].idx = static_cast<uint32_t>(i);
+                r[j].key = brick_key(p, r[j].x);
+            } else {
+                r[j] = p.rec_in[i];
+            }
+            digit[j] = PASS == 0 ? (r[j].key & 0xFFu) : ((r[j].key >> 8) & 0xFFu);
+            rank[j] = atomicAdd(&cnt[digit[j]], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t total;
+    lstart[threadIdx.x] = block_exclusive_scan(cnt[threadIdx.x], &total);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t i = first + j * kThreads + threadIdx.x;
+        if (i < p.npts) stage[lstart[digit[j]] + rank[j]] = r[j];
+    }
+    __syncthreads();
+    const int n = static_cast<int>(min<int64_t>(kChunk, p.npts - first));
+    for (int pos = threadIdx.x; pos < n; pos += kThreads) {
+        const BinRecord rec = stage[pos];
+        const uint32_t d = PASS == 0 ? (rec.key & 0xFFu) : ((rec.key >> 8) & 0xFFu);
+        const uint32_t slot = gbase[d] + (pos - lstart[d]);
+        p.rec_out[slot] = rec;
+        if constexpr (PASS == 0) p.key_out[slot] = rec.key;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                               uint32_t* __restrict__ tile_sums) {
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    uint32_t s = 0;
+    for (int j = 0; j < kScanTile / kThreads; ++j) {
+        const int64_t i = first + j * kThreads + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t total;
+    block_exclusive_scan(s, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the tile sums in place (ntiles <= kScanTile).
+__global__ __launch_bounds__(kThreads) void scan_tiles_kernel(uint32_t* __restrict__ sums, int64_t ntiles) {
+    constexpr int kPer = kScanTile / kThreads;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int64_t i = static_cast<int64_t>(threadIdx.x) * kPer + j;
+        v[j] = i < ntiles ? sums[i] : 0u;
+        s += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, &total);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int64_t i = static_cast<int64_t>(threadIdx.x) * kPer + j;
+        if (i < ntiles) sums[i] = run;
+        run += v[j];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                              const uint32_t* __restrict__ tile_offs,
+                                                              uint32_t* __restrict__ out) {
+    constexpr int kPer = kScanTile / kThreads;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kPer;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        v[j] = first + j < n ? in[first + j] : 0u;
+        s += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(s, &total) + tile_offs[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (first + j < n) out[first + j] = run;
+        run += v[j];
+    }
+}
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t* tile_sums, int64_t n, hipStream_t s) {
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (ntiles > kScanTile) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s, in, n,
+                       tile_sums);
+    hipLaunchKernelGGL(scan_tiles_kernel, dim3(1), dim3(kThreads), 0, s, tile_sums, ntiles);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(ntiles)), dim3(kThreads), 0, s, in, n,
+                       tile_sums, out);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int64_t bin_chunk_points() { return kChunk; }
+
+int64_t bin_hist_entries(int64_t npts) { return kRadix * ((npts + kChunk - 1) / kChunk); }
+
+int64_t bin_scan_tiles(int64_t npts) { return (bin_hist_entries(npts) + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_binning(BinParams p, uint32_t* hist, uint32_t* offs, uint32_t* tile_sums, uint32_t* keys,
+                          BinRecord* rec_a, BinRecord* rec_b, hipStream_t s) {
+    p.nblocks = static_cast<int32_t>((p.npts + kChunk - 1) / kChunk);
+    const int64_t nh = static_cast<int64_t>(kRadix) * p.nblocks;
+    const dim3 grid(static_cast<unsigned>(p.nblocks)), block(kThreads);
+    // pass 0: caller coordinates -> rec_a (and its keys) by low key byte
+    p.rec_out = rec_a;
+    p.key_out = keys;
+    hipLaunchKernelGGL(bin_hist_kernel<0>, grid, block, 0, s, p, hist);
+    hipError_t e = exclusive_scan(hist, offs, tile_sums, nh, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_scatter_kernel<0>, grid, block, 0, s, p, offs);
+    // pass 1: rec_a -> rec_b by high key byte
+    p.rec_in = rec_a;
+    p.key_in = keys;
+    p.rec_out = rec_b;
+    hipLaunchKernelGGL(bin_hist_kernel<1>, grid, block, 0, s, p, hist);
+    e = exclusive_scan(hist, offs, tile_sums, nh, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bin_scatter_kernel<1>, grid, block, 0, s, p, offs);
+    return hipGetLastError();
+}
+
+}  // namespace intp_amd

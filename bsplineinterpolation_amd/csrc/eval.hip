@@ -19,6 +19,8 @@
 // (contiguous) axis first.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace intp_amd {
@@ -418,10 +420,20 @@ __device__ __forceinline__ double contract_single(const double* __restrict__ c, 
 // kernels
 // ---------------------------------------------------------------------------
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Block b is remapped so that XCD x processes a contiguous range of
+// chunks: with grouped points, one L2 then holds each brick patch.
+__device__ __forceinline__ int64_t block_index() {
+    constexpr int64_t kXcd = 8;
+    int64_t b = blockIdx.x;
+    const int64_t per = static_cast<int64_t>(gridDim.x) / kXcd;
+    if (b < per * kXcd) b = (b % kXcd) * per + b / kXcd;
+    return b;
+}
+
 template <int D, int K, bool GRAD, bool LASTC>
-__global__ __launch_bounds__(kBlock) void eval_general_kernel(EvalParams p) {
-    const int64_t pt = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (pt >= p.npts) return;
+__device__ __forceinline__ void general_point(const EvalParams& p, int64_t pt) {
+    const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
     DimW<K, int64_t> dd[D];
     bool zero = false;
 #pragma unroll
@@ -435,10 +447,40 @@ __global__ __launch_bounds__(kBlock) void eval_general_kernel(EvalParams p) {
         contract_grad<D, K, LASTC>(p.ctrl, dd, res);
 #pragma unroll
         for (int s = 0; s <= D; ++s)
-            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + pt * p.os_pt] = res[s];
+            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
     } else {
         const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
-        p.out[p.out_index * p.os_out + pt * p.os_pt] = zero ? 0.0 : v;
+        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : v;
+    }
+}
+
+template <int D, int K, bool GRAD, bool LASTC>
+__global__ __launch_bounds__(kBlock) void eval_general_kernel(EvalParams p) {
+    const int64_t pt = block_index() * kBlock + threadIdx.x;
+    if (pt < p.npts) general_point<D, K, GRAD, LASTC>(p, pt);
+}
+
+template <int D, int K, bool GRAD, bool LASTC>
+__device__ __forceinline__ void uniform_point(const EvalParams& p, const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
+                                              int64_t pt) {
+    const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
+    DimW<K, uint32_t> dd[D];
+    bool zero = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int r = GRAD ? 1 : p.order[d];
+        zero = zero || r > K;
+        uniform_weights<K, GRAD>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+    }
+    if constexpr (GRAD) {
+        double res[D + 1];
+        contract_grad<D, K, LASTC>(p.ctrl, dd, res);
+#pragma unroll
+        for (int s = 0; s <= D; ++s)
+            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
+    } else {
+        const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
+        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : v;
     }
 }
 
@@ -453,46 +495,27 @@ __global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
         for (int i = threadIdx.x; i < n; i += kBlock) s_poly[d][i] = p.dim[d].poly[i];
     }
     __syncthreads();
-    const int64_t pt = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-    if (pt >= p.npts) return;
-    DimW<K, uint32_t> dd[D];
-    bool zero = false;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int r = GRAD ? 1 : p.order[d];
-        zero = zero || r > K;
-        uniform_weights<K, GRAD>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
-    }
-    if constexpr (GRAD) {
-        double res[D + 1];
-        contract_grad<D, K, LASTC>(p.ctrl, dd, res);
-#pragma unroll
-        for (int s = 0; s <= D; ++s)
-            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + pt * p.os_pt] = res[s];
-    } else {
-        const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
-        p.out[p.out_index * p.os_out + pt * p.os_pt] = zero ? 0.0 : v;
-    }
+    const int64_t pt = block_index() * kBlock + threadIdx.x;
+    if (pt < p.npts) uniform_point<D, K, GRAD, LASTC>(p, s_poly, pt);
+}
+
+template <void (*KF)(EvalParams)>
+hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
+    const int64_t blocks = (p.npts + kBlock - 1) / kBlock;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(KF, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+    return hipGetLastError();
 }
 
 template <int D, int K, bool LASTC>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
-    const int64_t blocks = (p.npts + kBlock - 1) / kBlock;
-    if (blocks == 0) return hipSuccess;
-    if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-    const dim3 g(static_cast<unsigned>(blocks)), b(kBlock);
     if (uniform) {
-        if (grad)
-            hipLaunchKernelGGL((eval_uniform_kernel<D, K, true, LASTC>), g, b, 0, s, p);
-        else
-            hipLaunchKernelGGL((eval_uniform_kernel<D, K, false, LASTC>), g, b, 0, s, p);
-    } else {
-        if (grad)
-            hipLaunchKernelGGL((eval_general_kernel<D, K, true, LASTC>), g, b, 0, s, p);
-        else
-            hipLaunchKernelGGL((eval_general_kernel<D, K, false, LASTC>), g, b, 0, s, p);
+        if (grad) return launch_chunks<eval_uniform_kernel<D, K, true, LASTC>>(p, s);
+        return launch_chunks<eval_uniform_kernel<D, K, false, LASTC>>(p, s);
     }
-    return hipGetLastError();
+    if (grad) return launch_chunks<eval_general_kernel<D, K, true, LASTC>>(p, s);
+    return launch_chunks<eval_general_kernel<D, K, false, LASTC>>(p, s);
 }
 
 template <int D, int K>

@@ -47,6 +47,35 @@ struct EvalParams {
     int32_t order[kMaxDim];
     int32_t out_index;
     int* bad;  // set to 1 if a clamped coordinate is outside range (or null)
+    // grouped evaluation: point pt writes to output position
+    // perm[pt * perm_stride] (null: position pt)
+    const uint32_t* perm;
+    int64_t perm_stride;
+};
+
+// A query point carried through the grouping passes (32 bytes).
+struct BinRecord {
+    double x[kMaxDim];
+    uint32_t idx;  // position in the caller's arrays
+    uint32_t key;  // brick id
+};
+
+struct BinDim {
+    double lo, period, inv_period, inv_dx;
+    int32_t periodic, ncell, brick_shift, nbrick;
+};
+
+struct BinParams {
+    BinDim dim[kMaxDim];
+    int32_t dim_count;
+    int32_t nblocks;
+    const double* coords;
+    int64_t cs_dim, cs_pt;
+    int64_t npts;
+    const BinRecord* rec_in;
+    BinRecord* rec_out;
+    const uint32_t* key_in;  // pass 1: keys of rec_in, dense
+    uint32_t* key_out;       // pass 0: keys of rec_out, dense
 };
 
 // Line solves along one axis of a C-order table viewed as
@@ -76,6 +105,13 @@ struct ShiftParams {
 hipError_t launch_eval_grad(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_eval_single(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s);
+
+// Grouping of query points by brick (bin.hip): scratch sizes and launch.
+int64_t bin_chunk_points();
+int64_t bin_hist_entries(int64_t npts);
+int64_t bin_scan_tiles(int64_t npts);
+hipError_t launch_binning(BinParams p, uint32_t* hist, uint32_t* offs, uint32_t* tile_sums, uint32_t* keys,
+                          BinRecord* rec_a, BinRecord* rec_b, hipStream_t s);
 hipError_t launch_shift_copy(const ShiftParams& p, hipStream_t s);
 
 }  // namespace intp_amd

@@ -50,6 +50,8 @@ typedef enum {
 /* eval flags */
 #define INTP_EVAL_BOUNDS_CHECK 1u /* at() / derivative_at() semantics     */
 #define INTP_EVAL_HOST_MEM 2u     /* coords and outputs are host memory   */
+#define INTP_EVAL_UNGROUPED 4u    /* evaluate in the caller's point order:
+                                     no grouping of large batches by brick */
 
 typedef struct intp_ctx_s* intp_ctx;
 typedef struct intp_template_s* intp_template;
@@ -147,7 +149,12 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
  * coordinates of clamped dimensions extrapolate the boundary polynomial,
  * periodic ones wrap; with INTP_EVAL_BOUNDS_CHECK the call returns
  * INTP_E_DOMAIN (and synchronizes `stream`) if any clamped coordinate is
- * outside range(d).  Orders above the spline order give 0.              */
+ * outside range(d).  Orders above the spline order give 0.
+ * Large batches on uniform 2D/3D grids are first grouped by brick of the
+ * control-point table (two counting-sort passes in scratch device memory
+ * from the stream-ordered allocator) so that each workgroup gathers from a
+ * cache-resident patch; results are written to the caller's positions and
+ * do not depend on the grouping.                                          */
 int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
               int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
               uint32_t flags, void* stream);

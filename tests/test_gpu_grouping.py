@@ -1,0 +1,78 @@
+"""Grouped evaluation (large batches bucketed by brick before evaluating)
+against the caller-order path and the oracle.
+
+Each point runs the same arithmetic in both paths, so the results must be
+bit-identical; a subsample is also checked against the oracle (1e-12)."""
+import math
+
+import numpy as np
+import pytest
+
+from cases import rel_err
+
+pytestmark = pytest.mark.gpu
+
+UNGROUPED = 4
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import bsplineinterpolation_amd as amd
+    return amd
+
+
+def _eval_both(g, q, orders):
+    import torch
+    qd = torch.from_numpy(q).cuda()
+    a = g.eval(qd, orders).cpu().numpy()
+    # same call, caller order
+    import ctypes as C
+    from bsplineinterpolation_amd import _lib
+    nout = len(orders)
+    D = g.dim
+    out = torch.empty((nout, q.shape[1]), dtype=torch.float64, device="cuda")
+    op = np.ascontiguousarray(np.asarray(orders, np.int32))
+    dp = C.POINTER(C.c_double)
+    _lib.check(_lib.load().intp_eval(g.h, q.shape[1], C.cast(qd.data_ptr(), dp), q.shape[1], 1, nout,
+                                     op.ctypes.data_as(C.POINTER(C.c_int32)), C.cast(out.data_ptr(), dp),
+                                     q.shape[1], 1, UNGROUPED, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return a, out.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape,k,per,npts", [
+    ((96, 80, 64), 3, [True, False, False], (1 << 21) + 3),
+    ((40, 33, 50), 5, [False, True, True], (1 << 20)),
+    ((300, 200), 4, [True, True], (1 << 20) + 77),
+    ((257, 129), 3, [False, False], (1 << 20)),
+])
+def test_gpu_grouped_equals_ungrouped(amd, orc, shape, k, per, npts):
+    rng = np.random.default_rng(sum(shape) + k)
+    f = rng.standard_normal(shape)
+    D = len(shape)
+    rg = [(-1.0 + 0.1 * d, 1.5 + 0.2 * d) for d in range(D)]
+    g = amd.InterpolationFunction(k, f, periodic=per, ranges=rg)
+    span = np.array([r[1] - r[0] for r in rg])
+    lo = np.array([r[0] for r in rg])
+    q = (lo[:, None] + span[:, None] * rng.uniform(-0.1, 1.1, (D, npts)))
+    orders = [[0] * D] + [list(e) for e in np.eye(D, dtype=int)] + [[2] * D]
+    grouped, plain = _eval_both(g, q, orders)
+    assert np.array_equal(grouped, plain)
+    o = orc.OracleSpline(k, f, periodic=per, ranges=rg)
+    idx = rng.choice(npts, 4096, replace=False)
+    want = o.eval(np.ascontiguousarray(q[:, idx].T), orders, nthreads=8)
+    for i in range(len(orders)):
+        assert rel_err(grouped[i, idx], want[i]) < 1e-12
+
+
+def test_gpu_grouped_host_buffers_and_bounds(amd):
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((64, 48, 40))
+    g = amd.InterpolationFunction(3, f, ranges=[(0, 1), (0, 1), (0, 1)])
+    q = rng.uniform(0, 1, ((1 << 20) + 5, 3))
+    host = g.eval(q, [[0, 0, 0], [0, 1, 0]])
+    import torch
+    dev = g.eval(torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), [[0, 0, 0], [0, 1, 0]]).cpu().numpy()
+    assert np.array_equal(host, dev)
+    q[777, 1] = 1.5
+    with pytest.raises(amd.DomainError):
+        g.eval(q, bounds=True)
