@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--queries", type=int, default=1 << 26)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="copy 256 MiB once before timing (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU layout is described in DESIGN.md, "
@@ -112,8 +114,10 @@ def main():
     w["nq"] = nq
     dev = torch.device("cuda:0")
     f_d = torch.from_numpy(w["f"]).to(dev)
-    q_d = torch.from_numpy(w["queries"]).to(dev)  # SoA [3, nq]
-    out_d = torch.empty((4, nq), dtype=torch.float64, device=dev)
+    # one record per query, like the reference's per-point calls:
+    # coordinates [nq, 3] in, (value, d/dx, d/dy, d/dz) [nq, 4] out
+    q_d = torch.from_numpy(np.ascontiguousarray(w["queries"].T)).to(dev)
+    out_d = torch.empty((nq, 4), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     ctx = amd.Context(0)
     per, rg, orders = w["periodic"], w["ranges"], w["orders"]
@@ -125,11 +129,17 @@ def main():
         s = amd.InterpolationFunction(3, f_d, periodic=per, ranges=rg, ctx=ctx)
         if ev is not None:
             ev[1].record(stream)
-        s.eval(q_d, orders, out=out_d)
+        s.eval(q_d, orders, out=out_d.t())
         if ev is not None:
             ev[2].record(stream)
         return s
 
+    if args.calibration_copy:
+        a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
     keep = None
     for _ in range(args.warmup):
@@ -193,7 +203,7 @@ def main():
     if not args.no_cpu:
         m = min(args.cpu_sample, nq)
         print("bench: CPU baseline (sample %d points)" % m, file=sys.stderr, flush=True)
-        sample = out_d[:, :m].cpu().numpy()
+        sample = out_d[:m].t().cpu().numpy()
         cb, errs = cpu_baseline(w, sample, m, min(1 << 15, m))
         result["cpu_baseline"] = cb
         result["speedup_vs_cpu"] = value / cb["value"]

@@ -206,6 +206,7 @@ class InterpolationFunction:
                 npts, cs_dim, cs_pt = pts.shape[0], 1, D
             if out is None:
                 out = torch.empty((nout, npts), dtype=torch.float64, device=pts.device)
+            assert tuple(out.shape) == (nout, npts) and out.dtype == torch.float64 and out.is_cuda
             st = _stream_handle(stream) if stream is not None else _current_stream()
             check(lib.intp_eval(self.h, npts, C.cast(pts.data_ptr(), _dp), cs_dim, cs_pt, nout, op,
                                 C.cast(out.data_ptr(), _dp), out.stride(0), out.stride(1), flags, st))
