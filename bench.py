@@ -14,8 +14,11 @@ per second over the whole step.
 
 The eval kernel dominates the step; its roofline entry uses the
 algorithmic bytes 2^26*(3*8 + 4*8) + 256^3*8 (coordinates in, outputs out,
-table read once) over its HIP-event-timed average duration on the stream it
-runs on.  The CPU baseline times the compiled reference
+table read once) over its average duration, timed by the library with HIP
+events around the kernel on the stream it runs on (INTP_EVAL_TIMED).  Its
+measured HBM traffic per launch comes from the committed rocprofv3
+FETCH_SIZE/WRITE_SIZE summary (profiles/bench_3d256_k3_hbm_traffic.md).
+The CPU baseline times the compiled reference
 (oracle/_ref/libintp_ref.so; the oracle restatement if that is absent):
 the constructor at full size with the reference's 8 pool workers, and
 evaluation of a sample of the same queries with one thread per host core.
@@ -35,6 +38,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+EVAL_KERNEL = "eval_uniform_kernel<3, 3, true, true>"
+TRAFFIC_PROFILE = os.path.join("profiles", "bench_3d256_k3_hbm_traffic.md")
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` (calibrated read + write) from the
+    committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == "`%s`" % kernel:
+            return (float(cells[4]) + float(cells[5])) * 1e9
+    return None
 
 
 def _cores() -> int:
@@ -129,7 +147,7 @@ def main():
         s = amd.InterpolationFunction(3, f_d, periodic=per, ranges=rg, ctx=ctx)
         if ev is not None:
             ev[1].record(stream)
-        s.eval(q_d, orders, out=out_d.t())
+        s.eval(q_d, orders, out=out_d.t(), timed=ev is not None)
         if ev is not None:
             ev[2].record(stream)
         return s
@@ -156,6 +174,10 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     build_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in events]))
     eval_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in events]))
+    timings = amd.eval_timings()
+    assert len(timings) == args.steps
+    group_ms = float(np.mean([t[0] for t in timings]))
+    kernel_ms = float(np.mean([t[1] for t in timings]))
 
     # control-point solve alone: template built once, interpolate timed
     tmpl = amd.InterpolationFunctionTemplate(3, (n, n, n), periodic=per, ranges=rg, ctx=ctx)
@@ -174,6 +196,7 @@ def main():
     b_eval = nq * (3 * 8 + 4 * 8) + N * 8
     b_solve = 2 * 3 * N * 8
     eval_gbs = b_eval / (eval_ms * 1e-3) / 1e9
+    kernel_gbs = b_eval / (kernel_ms * 1e-3) / 1e9
     solve_gbs = b_solve / (solve_ms * 1e-3) / 1e9
     value = nq / (ms_per_step * 1e-3)
 
@@ -193,11 +216,15 @@ def main():
         "config": {"workload": "3D %d^3 order-3 FP64 build (dim 0 periodic, dims 1-2 clamped) + %d queries x "
                                "(value, d/dx, d/dy, d/dz)" % (n, nq),
                    "mesh": [n, n, n], "order": 3, "periodic": per, "queries": nq, "outputs": 4},
-        "breakdown_ms": {"build_full_constructor": build_ms, "solve_only": solve_ms, "eval": eval_ms},
+        "breakdown_ms": {"build_full_constructor": build_ms, "solve_only": solve_ms, "eval_call": eval_ms,
+                         "eval_grouping": group_ms, "eval_kernel": kernel_ms},
         "solve": {"gbs": solve_gbs, "frac_hbm": solve_gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_solve},
-        "roofline": {"bound": "hbm", "kernel": "eval_grad_kernel<3,3>", "achieved": eval_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": eval_gbs / HBM_PEAK_GBS,
-                     "algorithmic_bytes": b_eval, "traffic": None},
+        "eval_call": {"gbs": eval_gbs, "frac_hbm": eval_gbs / HBM_PEAK_GBS, "algorithmic_bytes": b_eval},
+        "roofline": {"bound": "hbm", "kernel": EVAL_KERNEL, "achieved": kernel_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernel_gbs / HBM_PEAK_GBS,
+                     "algorithmic_bytes": b_eval, "avg_launch_ms": kernel_ms,
+                     "traffic": profiled_traffic(EVAL_KERNEL) if (n, nq) == (256, 1 << 26) else None,
+                     "traffic_source": TRAFFIC_PROFILE},
     }
 
     if not args.no_cpu:

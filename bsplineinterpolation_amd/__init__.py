@@ -18,7 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import DomainError, IntpError, check  # noqa: F401
 
-__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "DomainError", "IntpError"]
+__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "DomainError", "IntpError",
+           "eval_timings"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -182,17 +183,18 @@ class InterpolationFunction:
         return out
 
     # ---- evaluation ---------------------------------------------------
-    def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None):
+    def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None, timed: bool = False):
         """Batched evaluation.  ``points``: numpy [npts, D] (host) or a
         float64 torch tensor [D, npts] (SoA) / [npts, D] (AoS) on the
         device.  ``orders``: [nout, D] derivative orders (default: value).
-        Returns [nout, npts] (numpy or torch, like ``points``)."""
+        Returns [nout, npts] (numpy or torch, like ``points``).  ``timed``
+        records device events around the call (read with eval_timings())."""
         lib = _lib.load()
         D = self.dim
         orders = np.zeros((1, D), np.int32) if orders is None else \
             np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
         nout = orders.shape[0]
-        flags = _lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0
+        flags = (_lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0) | (_lib.INTP_EVAL_TIMED if timed else 0)
         op = orders.ctypes.data_as(C.POINTER(C.c_int32))
         if _is_torch(points):
             import torch
@@ -244,6 +246,19 @@ class InterpolationFunction:
         pts = np.asarray(coord, dtype=np.float64).reshape(-1, self.dim)
         v = self.eval(pts, r, bounds=True)[0]
         return float(v[0]) if v.size == 1 else v
+
+
+def eval_timings(capacity: int = 1 << 16):
+    """[(group_ms, kernel_ms)] of the evaluations made with ``timed=True``
+    on this thread since the last call (intp_eval_timings), oldest first;
+    waits for the last of them."""
+    lib = _lib.load()
+    g = np.zeros(capacity)
+    k = np.zeros(capacity)
+    n = C.c_int64()
+    check(lib.intp_eval_timings(g.ctypes.data_as(_dp), k.ctypes.data_as(_dp), capacity, C.byref(n)))
+    m = min(n.value, capacity)
+    return list(zip(g[:m].tolist(), k[:m].tolist()))
 
 
 class InterpolationFunctionTemplate:

@@ -26,13 +26,14 @@ INTP_MEM_HOST = 1
 INTP_EVAL_BOUNDS_CHECK = 1
 INTP_EVAL_HOST_MEM = 2
 INTP_EVAL_UNGROUPED = 4
+INTP_EVAL_TIMED = 8
 
 # every symbol include/intp_amd.h declares
 EXPORTS = (
     "intp_last_error", "intp_status_string", "intp_ctx_create", "intp_ctx_destroy",
     "intp_template_create", "intp_template_destroy", "intp_interpolate", "intp_build",
     "intp_spline_from_knots", "intp_spline_retain", "intp_spline_release", "intp_spline_get_info",
-    "intp_spline_download_ctrl", "intp_spline_download_knots", "intp_eval",
+    "intp_spline_download_ctrl", "intp_spline_download_knots", "intp_eval", "intp_eval_timings",
 )
 
 
@@ -87,6 +88,7 @@ def load(path: str = LIB_PATH):
     lib.intp_spline_download_knots.argtypes = [vp, C.c_int32, dp, C.c_int64]
     lib.intp_eval.argtypes = [vp, C.c_int64, dp, C.c_int64, C.c_int64, C.c_int32, i32p, dp, C.c_int64,
                               C.c_int64, C.c_uint32, vp]
+    lib.intp_eval_timings.argtypes = [dp, dp, C.c_int64, i64p]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

@@ -2,6 +2,7 @@
 // handles, device memory, argument checks and kernel dispatch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstddef>
 #include <cmath>
@@ -9,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/intp_amd.h"
@@ -55,56 +57,212 @@ class DeviceScope {
     bool ok_ = false;
 };
 
-struct DevArray {
-    double* p = nullptr;
-    size_t count = 0;
+// ---- device memory lifetimes -------------------------------------------
+//
+// Every device buffer comes from the stream-ordered allocator (the
+// device's default memory pool, kept cached up to kPoolKeepBytes) and is
+// released with hipFreeAsync on a per-device release stream that first
+// waits for every recorded use of the buffer.  Nothing on the build or
+// evaluation path calls hipMalloc/hipFree, which synchronize the device.
+
+constexpr int kMaxDevices = 64;
+constexpr uint64_t kPoolKeepBytes = uint64_t{16} << 30;
+
+struct DeviceRuntime {
+    bool ready = false;
+    hipStream_t release = nullptr;  // frees, ordered after recorded uses
+    hipStream_t setup = nullptr;    // uploads of calls that take no stream
 };
 
-// Knots, factors and reciprocal tables of one grid on one device; shared
-// by a template and every spline interpolated from it.
+// Per-device streams and pool settings, created on first use and kept
+// until the process ends.  Both streams are non-blocking with default
+// priority.
+DeviceRuntime* device_runtime(int device) {
+    static std::mutex mu;
+    static DeviceRuntime rt[kMaxDevices];
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    DeviceRuntime& r = rt[device];
+    if (!r.ready) {
+        DeviceScope scope(device);
+        if (!scope.ok()) return nullptr;
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = 0;
+            if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess &&
+                keep < kPoolKeepBytes) {
+                keep = kPoolKeepBytes;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        }
+        if (!r.release && hipStreamCreateWithFlags(&r.release, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (!r.setup && hipStreamCreateWithFlags(&r.setup, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        r.ready = true;
+    }
+    return &r;
+}
+
+// The streams an object's device memory was used on, each with an event
+// recorded after its latest use there.
+class UseTracker {
+   public:
+    UseTracker() = default;
+    UseTracker(const UseTracker&) = delete;
+    UseTracker& operator=(const UseTracker&) = delete;
+    ~UseTracker() {
+        for (const Use& u : uses_) (void)hipEventDestroy(u.event);
+    }
+
+    hipError_t record(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (const Use& u : uses_)
+            if (u.stream == s) return hipEventRecord(u.event, s);
+        if (uses_.size() >= kMaxStreams) {
+            // many distinct streams: settle the old ones instead of growing
+            for (const Use& u : uses_) {
+                (void)hipEventSynchronize(u.event);
+                (void)hipEventDestroy(u.event);
+            }
+            uses_.clear();
+        }
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        uses_.push_back({s, ev});
+        return hipEventRecord(ev, s);
+    }
+
+    // Makes `s` wait for every recorded use.
+    void order_before(hipStream_t s) {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (const Use& u : uses_) (void)hipStreamWaitEvent(s, u.event, 0);
+    }
+
+    hipError_t wait_host() {
+        std::lock_guard<std::mutex> lock(mu_);
+        for (const Use& u : uses_) {
+            const hipError_t e = hipEventSynchronize(u.event);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+
+   private:
+    struct Use {
+        hipStream_t stream;
+        hipEvent_t event;
+    };
+    static constexpr size_t kMaxStreams = 16;
+    std::mutex mu_;
+    std::vector<Use> uses_;
+};
+
+// Frees `p` (from hipMallocAsync) once every use recorded in `uses` is
+// done, without blocking the host.
+void release_after(int device, void* p, UseTracker& uses) {
+    if (!p) return;
+    DeviceScope scope(device);
+    DeviceRuntime* rt = device_runtime(device);
+    if (rt) {
+        uses.order_before(rt->release);
+        if (hipFreeAsync(p, rt->release) == hipSuccess) return;
+    }
+    (void)uses.wait_host();
+    (void)hipFree(p);
+}
+
+// Knots, factors, reciprocal tables and basis polynomials of one grid on
+// one device, packed into one buffer; shared by a template and every
+// spline interpolated from it.
 struct GridData {
     int device = 0;
     int dim = 0, order = 0;
     DimSetup setup[kMaxDim];
-    DevArray knots[kMaxDim], recip[kMaxDim];
-    DevArray lower[kMaxDim], upper[kMaxDim], diag[kMaxDim], bottom[kMaxDim], right[kMaxDim];
-    DevArray poly[kMaxDim];
+    // views into `dev` (nullptr where the table is empty)
+    const double* knots[kMaxDim] = {};
+    const double* recip[kMaxDim] = {};
+    const double* lower[kMaxDim] = {};
+    const double* upper[kMaxDim] = {};
+    const double* diag[kMaxDim] = {};
+    const double* bottom[kMaxDim] = {};
+    const double* right[kMaxDim] = {};
+    const double* poly[kMaxDim] = {};
     bool has_factors = false;
     int64_t total = 1;
+    std::vector<double> image;  // host copy of `dev`, the source of its upload
+    double* dev = nullptr;
+    hipEvent_t uploaded = nullptr;
+    UseTracker uses;
 
     ~GridData() {
-        DeviceScope scope(device);
-        for (int d = 0; d < kMaxDim; ++d)
-            for (DevArray* a : {&knots[d], &recip[d], &lower[d], &upper[d], &diag[d], &bottom[d], &right[d],
-                                &poly[d]})
-                if (a->p) (void)hipFree(a->p);
+        release_after(device, dev, uses);
+        if (uploaded) {
+            // the upload reads `image`, which is freed after this body
+            DeviceScope scope(device);
+            (void)hipEventSynchronize(uploaded);
+            (void)hipEventDestroy(uploaded);
+        }
     }
 };
 
-int upload(const std::vector<double>& v, DevArray& out) {
-    out.count = v.size();
-    if (v.empty()) return INTP_OK;
-    HIP_TRY(hipMalloc(&out.p, v.size() * sizeof(double)), "hipMalloc");
-    HIP_TRY(hipMemcpy(out.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+// Packs every table of `g` (each 256-byte aligned) into one image and
+// uploads it with one asynchronous copy on `stream`.
+int upload_grid(GridData& g, hipStream_t stream) {
+    struct Item {
+        const std::vector<double>* src;
+        const double** view;
+    };
+    std::vector<Item> items;
+    for (int d = 0; d < g.dim; ++d) {
+        DimSetup& s = g.setup[d];
+        items.push_back({&s.knots, &g.knots[d]});
+        items.push_back({&s.recip, &g.recip[d]});
+        items.push_back({&s.poly, &g.poly[d]});
+        if (g.has_factors) {
+            items.push_back({&s.lu.lower, &g.lower[d]});
+            items.push_back({&s.lu.upper, &g.upper[d]});
+            items.push_back({&s.lu.diag, &g.diag[d]});
+            items.push_back({&s.lu.bottom, &g.bottom[d]});
+            items.push_back({&s.lu.right, &g.right[d]});
+        }
+    }
+    std::vector<size_t> offset;
+    size_t total = 0;
+    for (const Item& it : items) {
+        offset.push_back(total);
+        total += (it.src->size() + 31) / 32 * 32;
+    }
+    g.image.assign(total, 0.0);
+    for (size_t i = 0; i < items.size(); ++i)
+        std::copy(items[i].src->begin(), items[i].src->end(), g.image.begin() + offset[i]);
+    const size_t bytes = total * sizeof(double);
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&g.dev), bytes, stream), "hipMallocAsync (grid tables)");
+    HIP_TRY(hipMemcpyAsync(g.dev, g.image.data(), bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+    HIP_TRY(g.uses.record(stream), "hipEventRecord");
+    HIP_TRY(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming), "hipEventCreate");
+    HIP_TRY(hipEventRecord(g.uploaded, stream), "hipEventRecord");
+    for (size_t i = 0; i < items.size(); ++i) *items[i].view = items[i].src->empty() ? nullptr : g.dev + offset[i];
     return INTP_OK;
 }
 
-int upload_grid(GridData& g) {
-    for (int d = 0; d < g.dim; ++d) {
-        DimSetup& s = g.setup[d];
-        int rc;
-        if ((rc = upload(s.knots, g.knots[d])) != INTP_OK) return rc;
-        if ((rc = upload(s.recip, g.recip[d])) != INTP_OK) return rc;
-        if ((rc = upload(s.poly, g.poly[d])) != INTP_OK) return rc;
-        if (g.has_factors) {
-            if ((rc = upload(s.lu.lower, g.lower[d])) != INTP_OK) return rc;
-            if ((rc = upload(s.lu.upper, g.upper[d])) != INTP_OK) return rc;
-            if ((rc = upload(s.lu.diag, g.diag[d])) != INTP_OK) return rc;
-            if ((rc = upload(s.lu.bottom, g.bottom[d])) != INTP_OK) return rc;
-            if ((rc = upload(s.lu.right, g.right[d])) != INTP_OK) return rc;
+// ---- INTP_EVAL_TIMED: events around the grouping and the kernels ---------
+
+struct TimedCall {
+    int device;
+    hipEvent_t ev[3];  // before grouping, before the eval kernels, after them
+};
+
+thread_local std::vector<TimedCall> t_timed;
+thread_local std::vector<std::pair<int, hipEvent_t>> t_spare_events;
+
+hipError_t timing_event(int device, hipEvent_t* out) {
+    for (size_t i = 0; i < t_spare_events.size(); ++i)
+        if (t_spare_events[i].first == device) {
+            *out = t_spare_events[i].second;
+            t_spare_events.erase(t_spare_events.begin() + static_cast<std::ptrdiff_t>(i));
+            return hipSuccess;
         }
-    }
-    return INTP_OK;
+    return hipEventCreate(out);
 }
 
 }  // namespace
@@ -121,11 +279,11 @@ struct intp_spline_s {
     std::atomic<int> refs{1};
     std::shared_ptr<GridData> grid;
     double* ctrl = nullptr;
+    UseTracker uses;  // streams that read `ctrl` (solve and every eval)
     ~intp_spline_s() {
-        if (ctrl) {
-            DeviceScope scope(grid->device);
-            (void)hipFree(ctrl);
-        }
+        // ordered on the release stream before the grid's own release,
+        // which runs when `grid` drops its last reference below
+        release_after(grid->device, ctrl, uses);
     }
 };
 
@@ -141,7 +299,11 @@ int check_dims(int dim, int order) {
     return INTP_OK;
 }
 
-int make_grid(intp_ctx ctx, const intp_grid_desc* desc, std::shared_ptr<GridData>& out) {
+// Knots, factored collocation matrices and basis tables of `desc`,
+// uploaded on `stream` (on the device's setup stream, waited for, when
+// `stream` is null: a template outlives any one stream).
+int make_grid(intp_ctx ctx, const intp_grid_desc* desc, hipStream_t stream, bool own_stream,
+              std::shared_ptr<GridData>& out) {
     if (!ctx || !desc) return fail(INTP_E_INVALID, "null argument");
     int rc = check_dims(desc->dim, desc->order);
     if (rc != INTP_OK) return rc;
@@ -158,7 +320,11 @@ int make_grid(intp_ctx ctx, const intp_grid_desc* desc, std::shared_ptr<GridData
     }
     DeviceScope scope(ctx->device);
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
-    if ((rc = upload_grid(*g)) != INTP_OK) return rc;
+    DeviceRuntime* rt = device_runtime(ctx->device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    if (own_stream) stream = rt->setup;
+    if ((rc = upload_grid(*g, stream)) != INTP_OK) return rc;
+    if (own_stream) HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     out = std::move(g);
     return INTP_OK;
 }
@@ -195,11 +361,11 @@ int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_
         for (int d = 0; d < a; ++d) outer *= g.setup[d].n;
         sp.inner = inner;
         sp.nlines = inner * outer;
-        sp.lower = g.lower[a].p;
-        sp.upper = g.upper[a].p;
-        sp.diag = g.diag[a].p;
-        sp.bottom = g.bottom[a].p;
-        sp.right = g.right[a].p;
+        sp.lower = g.lower[a];
+        sp.upper = g.upper[a];
+        sp.diag = g.diag[a];
+        sp.bottom = g.bottom[a];
+        sp.right = g.right[a];
         hipError_t e = launch_solve(s.lu.p, s.lu.cyclic, sp, stream);
         if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
         HIP_TRY(e, "solve launch");
@@ -210,8 +376,8 @@ int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_
 DimParams dim_params(const GridData& g, int d) {
     const DimSetup& s = g.setup[d];
     DimParams p{};
-    p.knots = g.knots[d].p;
-    p.recip = g.recip[d].p;
+    p.knots = g.knots[d];
+    p.recip = g.recip[d];
     p.n = s.n;
     int64_t stride = 1;
     for (int e = d + 1; e < g.dim; ++e) stride *= g.setup[e].n;
@@ -228,7 +394,7 @@ DimParams dim_params(const GridData& g, int d) {
     p.xmax = s.xmax;
     p.dx = s.dx;
     p.half = s.half;
-    p.poly = g.poly[d].p;
+    p.poly = g.poly[d];
     p.nleft = s.nleft;
     p.nright = s.nright;
     p.nclass = s.nclass;
@@ -279,7 +445,7 @@ BinParams bin_params(const GridData& g) {
 bool uniform_path(const GridData& g) {
     if (g.total >= (int64_t{1} << 32)) return false;
     for (int d = 0; d < g.dim; ++d)
-        if (!g.setup[d].uniform || g.poly[d].p == nullptr) return false;
+        if (!g.setup[d].uniform || g.poly[d] == nullptr) return false;
     return true;
 }
 
@@ -318,7 +484,7 @@ int intp_ctx_destroy(intp_ctx ctx) {
 int intp_template_create(intp_ctx ctx, const intp_grid_desc* desc, intp_template* out) {
     if (!out) return fail(INTP_E_INVALID, "null output handle");
     std::shared_ptr<GridData> g;
-    const int rc = make_grid(ctx, desc, g);
+    const int rc = make_grid(ctx, desc, nullptr, true, g);
     if (rc != INTP_OK) return rc;
     *out = new intp_template_s{std::move(g)};
     return INTP_OK;
@@ -338,7 +504,7 @@ int intp_interpolate(intp_template t, const double* values, int values_mem, intp
     const size_t bytes = static_cast<size_t>(g.total) * sizeof(double);
     std::unique_ptr<intp_spline_s> s(new intp_spline_s());
     s->grid = t->grid;
-    HIP_TRY(hipMalloc(&s->ctrl, bytes), "hipMalloc (control points)");
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&s->ctrl), bytes, stream), "hipMallocAsync (control points)");
     const double* src = values;
     double* staged = nullptr;
     if (values_mem == INTP_MEM_HOST) {
@@ -349,13 +515,18 @@ int intp_interpolate(intp_template t, const double* values, int values_mem, intp
         return fail(INTP_E_INVALID, "values_mem must be INTP_MEM_HOST or INTP_MEM_DEVICE");
     }
     int rc = solve_into(g, src, s->ctrl, stream);
+    // the spline's release waits for the solve; the grid's for its use here
+    const hipError_t er = s->uses.record(stream);
+    const hipError_t eg = t->grid->uses.record(stream);
+    if (rc == INTP_OK && er != hipSuccess) rc = hip_fail(er, "hipEventRecord");
+    if (rc == INTP_OK && eg != hipSuccess) rc = hip_fail(eg, "hipEventRecord");
     if (staged) {
         if (rc == INTP_OK) {
             HIP_TRY(hipFreeAsync(staged, stream), "hipFreeAsync");
             HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
         } else {
+            (void)hipFreeAsync(staged, stream);
             (void)hipStreamSynchronize(stream);
-            (void)hipFree(staged);
         }
     }
     if (rc != INTP_OK) return rc;
@@ -367,7 +538,7 @@ int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, i
                void* stream) {
     if (!out) return fail(INTP_E_INVALID, "null output handle");
     intp_template_s t;
-    const int rc = make_grid(ctx, desc, t.grid);
+    const int rc = make_grid(ctx, desc, static_cast<hipStream_t>(stream), false, t.grid);
     if (rc != INTP_OK) return rc;
     return intp_interpolate(&t, values, values_mem, out, stream);
 }
@@ -391,12 +562,16 @@ int intp_spline_from_knots(intp_ctx ctx, int32_t dim, int32_t order, const int32
     }
     DeviceScope scope(ctx->device);
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
-    if ((rc = upload_grid(*g)) != INTP_OK) return rc;
+    DeviceRuntime* rt = device_runtime(ctx->device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    if ((rc = upload_grid(*g, rt->setup)) != INTP_OK) return rc;
     std::unique_ptr<intp_spline_s> s(new intp_spline_s());
     s->grid = std::move(g);
     const size_t bytes = static_cast<size_t>(s->grid->total) * sizeof(double);
-    HIP_TRY(hipMalloc(&s->ctrl, bytes), "hipMalloc (control points)");
-    HIP_TRY(hipMemcpy(s->ctrl, ctrl, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&s->ctrl), bytes, rt->setup), "hipMallocAsync (control points)");
+    HIP_TRY(hipMemcpyAsync(s->ctrl, ctrl, bytes, hipMemcpyHostToDevice, rt->setup), "hipMemcpyAsync");
+    HIP_TRY(s->uses.record(rt->setup), "hipEventRecord");
+    HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
     *out = s.release();
     return INTP_OK;
 }
@@ -434,8 +609,13 @@ int intp_spline_download_ctrl(intp_spline s, double* host, int64_t count) {
     if (!s || !host) return fail(INTP_E_INVALID, "null argument");
     if (count != s->grid->total) return fail(INTP_E_INVALID, "count must equal the number of control points");
     DeviceScope scope(s->grid->device);
-    HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    HIP_TRY(hipMemcpy(host, s->ctrl, static_cast<size_t>(count) * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    DeviceRuntime* rt = device_runtime(s->grid->device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    s->uses.order_before(rt->setup);
+    HIP_TRY(hipMemcpyAsync(host, s->ctrl, static_cast<size_t>(count) * sizeof(double), hipMemcpyDeviceToHost,
+                           rt->setup),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
     return INTP_OK;
 }
 
@@ -534,6 +714,18 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     }
     hipError_t e = hipSuccess;
     const bool fast = uniform_path(g);
+    TimedCall timed{g.device, {nullptr, nullptr, nullptr}};
+    const bool timing = (flags & INTP_EVAL_TIMED) != 0;
+    if (timing) {
+        for (int i = 0; i < 3 && e == hipSuccess; ++i) e = timing_event(g.device, &timed.ev[i]);
+        if (e == hipSuccess) e = hipEventRecord(timed.ev[0], stream);
+        if (e != hipSuccess) {
+            for (hipEvent_t ev : timed.ev)
+                if (ev) t_spare_events.emplace_back(g.device, ev);
+            cleanup();
+            return hip_fail(e, "timing events");
+        }
+    }
     BinRecord* rec_a = nullptr;
     BinRecord* rec_b = nullptr;
     uint32_t* bin_scratch = nullptr;
@@ -563,6 +755,8 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         if (e != hipSuccess) {
             free_bins();
             cleanup();
+            if (timing)
+                for (hipEvent_t ev : timed.ev) t_spare_events.emplace_back(g.device, ev);
             return hip_fail(e, "grouping query points");
         }
         p.coords = reinterpret_cast<const double*>(rec_b);
@@ -571,11 +765,25 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         p.perm = reinterpret_cast<const uint32_t*>(rec_b) + offsetof(BinRecord, idx) / sizeof(uint32_t);
         p.perm_stride = sizeof(BinRecord) / sizeof(uint32_t);
     }
-    if (any_grad) e = launch_eval_grad(D, g.order, fast, p, stream);
+    if (timing && e == hipSuccess) e = hipEventRecord(timed.ev[1], stream);
+    if (any_grad && e == hipSuccess) e = launch_eval_grad(D, g.order, fast, p, stream);
     for (size_t i = 0; e == hipSuccess && i < single.size(); ++i) {
         for (int d = 0; d < D; ++d) p.order[d] = orders[single[i] * D + d];
         p.out_index = single[i];
         e = launch_eval_single(D, g.order, fast, p, stream);
+    }
+    if (timing) {
+        if (e == hipSuccess) e = hipEventRecord(timed.ev[2], stream);
+        if (e == hipSuccess) {
+            t_timed.push_back(timed);
+        } else {
+            for (hipEvent_t ev : timed.ev) t_spare_events.emplace_back(g.device, ev);
+        }
+    }
+    {
+        // the spline's release waits for this call's kernels
+        const hipError_t eu = s->uses.record(stream);
+        if (e == hipSuccess) e = eu;
     }
     free_bins();
     if (e != hipSuccess) {
@@ -603,6 +811,30 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     if (host || dbad) HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (bad) return fail(INTP_E_DOMAIN, "Given coordinate out of interpolation function range!");
     return INTP_OK;
+}
+
+int intp_eval_timings(double* group_ms, double* kernel_ms, int64_t capacity, int64_t* count) {
+    if (!count || capacity < 0 || (capacity > 0 && (!group_ms || !kernel_ms)))
+        return fail(INTP_E_INVALID, "null argument or negative capacity");
+    int rc = INTP_OK;
+    int64_t n = 0;
+    for (const TimedCall& t : t_timed) {
+        DeviceScope scope(t.device);
+        float g_ms = 0.f, k_ms = 0.f;
+        hipError_t e = hipEventSynchronize(t.ev[2]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&g_ms, t.ev[0], t.ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&k_ms, t.ev[1], t.ev[2]);
+        if (e != hipSuccess && rc == INTP_OK) rc = hip_fail(e, "timing events");
+        if (n < capacity) {
+            group_ms[n] = g_ms;
+            kernel_ms[n] = k_ms;
+        }
+        ++n;
+        for (hipEvent_t ev : t.ev) t_spare_events.emplace_back(t.device, ev);
+    }
+    t_timed.clear();
+    *count = n;
+    return rc;
 }
 
 }  // extern "C"

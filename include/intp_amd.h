@@ -52,6 +52,9 @@ typedef enum {
 #define INTP_EVAL_HOST_MEM 2u     /* coords and outputs are host memory   */
 #define INTP_EVAL_UNGROUPED 4u    /* evaluate in the caller's point order:
                                      no grouping of large batches by brick */
+#define INTP_EVAL_TIMED 8u        /* record HIP events around this call's
+                                     grouping and its evaluation kernels,
+                                     read back with intp_eval_timings()   */
 
 typedef struct intp_ctx_s* intp_ctx;
 typedef struct intp_template_s* intp_template;
@@ -158,6 +161,15 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
 int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
               int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
               uint32_t flags, void* stream);
+
+/* Durations (ms) of the calls made with INTP_EVAL_TIMED on this thread
+ * since the last query, oldest first: group_ms[i] covers the grouping
+ * pass (0 when none ran), kernel_ms[i] the evaluation kernels alone, each
+ * measured with HIP events on the call's own stream.  Writes up to
+ * `capacity` entries, sets *count to the number of timed calls and
+ * forgets them; waits for the last timed call to finish.  No reference
+ * counterpart (a measurement hook for benchmarks and profiling).         */
+int intp_eval_timings(double* group_ms, double* kernel_ms, int64_t capacity, int64_t* count);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,90 @@
+"""Stream-ordered lifetimes and the evaluation timing hook.
+
+A spline released while its kernels are still queued must stay readable
+until they finish (its memory is freed on a release stream ordered after
+every recorded use); builds and evaluations on different streams must see
+each other's work when the caller orders the streams."""
+import numpy as np
+import pytest
+
+from cases import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import bsplineinterpolation_amd as amd
+    return amd
+
+
+def test_gpu_release_while_queued(amd, orc):
+    import torch
+    rng = np.random.default_rng(11)
+    shape = (120, 96, 80)
+    f = rng.standard_normal(shape)
+    rg = [(0.0, 1.0), (-1.0, 1.0), (0.0, 2.0)]
+    per = [True, False, False]
+    fd = torch.from_numpy(f).cuda()
+    q = np.stack([rng.uniform(r[0], r[1], 1 << 20) for r in rg])
+    qd = torch.from_numpy(q).cuda()
+    orders = [[0, 0, 0], [1, 0, 0], [0, 0, 1]]
+    side = torch.cuda.Stream()
+    outs = []
+    with torch.cuda.stream(side):
+        for _ in range(4):
+            g = amd.InterpolationFunction(3, fd, periodic=per, ranges=rg)
+            outs.append(g.eval(qd, orders, stream=side))
+            del g  # released with its build and eval still queued
+    side.synchronize()
+    ref = orc.OracleSpline(3, f, periodic=per, ranges=rg)
+    idx = rng.choice(q.shape[1], 2048, replace=False)
+    want = ref.eval(np.ascontiguousarray(q[:, idx].T), orders, nthreads=8)
+    for o in outs:
+        got = o.cpu().numpy()
+        assert np.array_equal(got, outs[0].cpu().numpy())
+        for i in range(len(orders)):
+            assert rel_err(got[i, idx], want[i]) < 1e-12
+
+
+def test_gpu_build_and_eval_on_two_streams(amd, orc):
+    import torch
+    rng = np.random.default_rng(12)
+    f = rng.standard_normal((200, 150))
+    rg = [(0.0, 3.0), (1.0, 2.0)]
+    fd = torch.from_numpy(f).cuda()
+    a, b = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(a):
+        g = amd.InterpolationFunction(4, fd, periodic=[False, True], ranges=rg)
+    b.wait_stream(a)
+    q = np.stack([rng.uniform(r[0], r[1], 5000) for r in rg])
+    with torch.cuda.stream(b):
+        got = g.eval(torch.from_numpy(q).cuda(), stream=b)
+    del g
+    b.synchronize()
+    want = orc.OracleSpline(4, f, periodic=[False, True], ranges=rg).eval(np.ascontiguousarray(q.T), nthreads=8)
+    assert rel_err(got.cpu().numpy()[0], want[0]) < 1e-12
+    # control points of a spline from a template, downloaded after release of the template
+    t = amd.InterpolationFunctionTemplate(3, (64, 64), ranges=rg)
+    s = t.interpolate(fd[:64, :64].contiguous())
+    del t
+    ctrl = s.control_points()
+    o = orc.OracleSpline(3, f[:64, :64].copy(), ranges=rg)
+    assert np.array_equal(ctrl, o.control_points())
+
+
+def test_gpu_eval_timings(amd):
+    import torch
+    rng = np.random.default_rng(13)
+    g = amd.InterpolationFunction(3, rng.standard_normal((64, 64, 64)), ranges=[(0, 1)] * 3)
+    assert amd.eval_timings() == []
+    small = torch.from_numpy(rng.uniform(0, 1, (3, 1000))).cuda()
+    large = torch.from_numpy(rng.uniform(0, 1, (3, 1 << 20))).cuda()
+    g.eval(small, timed=True)
+    g.eval(large, [[0, 0, 0], [1, 0, 0]], timed=True)
+    g.eval(small)  # not timed
+    t = amd.eval_timings()
+    assert len(t) == 2
+    assert t[0][0] < 0.05 and t[0][1] > 0.0  # small batch: no grouping pass
+    assert t[1][0] > 0.0 and t[1][1] > 0.0   # large batch: grouped
+    assert amd.eval_timings() == []
