@@ -329,29 +329,39 @@ int make_grid(intp_ctx ctx, const intp_grid_desc* desc, hipStream_t stream, bool
     return INTP_OK;
 }
 
-// Control-point solve of `values` (device) into `ctrl` (device):
-// periodic shift copy, then one line-solve pass per axis, last axis first
-// (InterpolationTemplate.hpp:448-580).
+// Control-point solve of `values` (device) into `ctrl` (device): one
+// line-solve pass per axis, last axis first (InterpolationTemplate.hpp:448-580).
+// The first pass reads `values` with the periodic shift of
+// InterpolationTemplate.hpp:452-462 applied; when its lines do not fit
+// the LDS-tiled kernel, a shift copy precedes an in-place first pass.
 int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_t stream) {
-    ShiftParams sh{};
-    sh.src = values;
-    sh.dst = ctrl;
-    sh.total = g.total;
-    sh.dim = g.dim;
+    const int last = g.dim - 1;
+    int64_t shift[kMaxDim] = {};
     bool any_shift = false;
     for (int d = 0; d < g.dim; ++d) {
-        sh.n[d] = g.setup[d].n;
-        sh.shift[d] = g.setup[d].periodic ? (g.order / 2) % g.setup[d].n : 0;
-        any_shift = any_shift || sh.shift[d] != 0;
+        shift[d] = g.setup[d].periodic ? (g.order / 2) % g.setup[d].n : 0;
+        any_shift = any_shift || shift[d] != 0;
     }
-    if (any_shift) {
-        HIP_TRY(launch_shift_copy(sh, stream), "shift copy launch");
-    } else if (values != ctrl) {
-        HIP_TRY(hipMemcpyAsync(ctrl, values, static_cast<size_t>(g.total) * sizeof(double), hipMemcpyDeviceToDevice,
-                               stream),
-                "hipMemcpyAsync");
+    const bool fused = solve_tile_lines(g.setup[last].n) > 0;
+    if (!fused) {
+        if (any_shift) {
+            ShiftParams sh{};
+            sh.src = values;
+            sh.dst = ctrl;
+            sh.total = g.total;
+            sh.dim = g.dim;
+            for (int d = 0; d < g.dim; ++d) {
+                sh.n[d] = g.setup[d].n;
+                sh.shift[d] = shift[d];
+            }
+            HIP_TRY(launch_shift_copy(sh, stream), "shift copy launch");
+        } else if (values != ctrl) {
+            HIP_TRY(hipMemcpyAsync(ctrl, values, static_cast<size_t>(g.total) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, stream),
+                    "hipMemcpyAsync");
+        }
     }
-    for (int a = g.dim - 1; a >= 0; --a) {
+    for (int a = last; a >= 0; --a) {
         const DimSetup& s = g.setup[a];
         SolveParams sp{};
         sp.x = ctrl;
@@ -366,6 +376,16 @@ int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_
         sp.diag = g.diag[a];
         sp.bottom = g.bottom[a];
         sp.right = g.right[a];
+        sp.lines_per_block = solve_tile_lines(s.n);
+        if (a == last && fused) {
+            sp.src = values;
+            sp.outer_dims = last;
+            for (int d = 0; d < last; ++d) {
+                sp.outer_n[d] = g.setup[d].n;
+                sp.outer_shift[d] = shift[d];
+            }
+            sp.shift_last = shift[last];
+        }
         hipError_t e = launch_solve(s.lu.p, s.lu.cyclic, sp, stream);
         if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
         HIP_TRY(e, "solve launch");

@@ -81,13 +81,22 @@ struct BinParams {
 // Line solves along one axis of a C-order table viewed as
 // (outer, n, inner): line (o, i) holds elements o*n*inner + j*inner + i.
 struct SolveParams {
-    double* x;
+    double* x;  // solved in place
+    // First pass only (last axis, inner == 1): read the right-hand sides
+    // from `src` with the periodic shift applied, element (i_0..i_{D-1}) of
+    // x taking source element ((i_d - shift_d) mod n_d); nullptr: in place.
+    const double* src;
     int64_t n, inner, nlines;
     const double* lower;   // n * bw
     const double* upper;   // n * bw
     const double* diag;    // n
     const double* bottom;  // bw * n (cyclic)
     const double* right;   // bw * n (cyclic)
+    int32_t outer_dims;            // dims before the solved (last) axis
+    int64_t outer_n[kMaxDim];      // their sizes
+    int64_t outer_shift[kMaxDim];  // their shifts
+    int64_t shift_last;            // shift along the solved axis
+    int32_t lines_per_block;       // LDS-tiled kernel: lines per workgroup (0: one lane per line from HBM)
 };
 
 struct ShiftParams {
@@ -105,6 +114,9 @@ struct ShiftParams {
 hipError_t launch_eval_grad(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_eval_single(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s);
+// Lines of length n that one LDS-tiled solve workgroup holds on the
+// current device (a power of two up to 64), 0 if a line does not fit.
+int solve_tile_lines(int64_t n);
 
 // Grouping of query points by brick (bin.hip): scratch sizes and launch.
 int64_t bin_chunk_points();

@@ -183,3 +183,21 @@ def test_gpu_highest_derivative_on_every_knot(amd, orc, k, per):
         want = o.eval(pts, [[r]])[0]
         got = g.eval(pts, [[r]])[0]
         assert rel_err(got, want) < TOL, r
+
+
+@pytest.mark.parametrize("shape,k,per", [
+    ((12000,), 3, [True]),        # line longer than an LDS tile: shift copy + in-place HBM solve
+    ((12000,), 4, [False]),
+    ((8, 10500), 5, [True, True]),
+    ((300, 40, 33), 3, [True, False, True]),   # tiles straddling outer lines (inner % lines != 0)
+    ((7, 257, 129), 6, [False, True, False]),
+])
+def test_gpu_solve_tiling_and_long_lines(amd, orc, shape, k, per):
+    """Control points bit-identical to the oracle for lines that fit an LDS
+    tile (first pass fused with the periodic shift) and lines that do not."""
+    rng = np.random.default_rng(sum(shape) + k)
+    f = rng.standard_normal(shape)
+    rg = [(0.0, 1.0 + d) for d in range(len(shape))]
+    o = orc.OracleSpline(k, f, periodic=per, ranges=rg)
+    g = amd.InterpolationFunction(k, f, periodic=per, ranges=rg)
+    assert np.array_equal(g.control_points(), o.control_points())
