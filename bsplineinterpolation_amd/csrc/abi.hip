@@ -424,39 +424,35 @@ DimParams dim_params(const GridData& g, int d) {
 // Batches at least this large are grouped by brick before evaluation.
 constexpr int64_t kGroupMinPoints = int64_t{1} << 20;
 
-// Brick of B^D control points per bin, B a power of two: the largest B
-// whose (B + order)^D patch stays within 16 KB (well inside a CU's L1),
-// coarsened until the brick key fits 16 bits.
-BinParams bin_params(const GridData& g) {
+// Bricks of B^D stencil starts, B a power of two: the largest B whose
+// (B + order)^D patch stays within 16 KB (the LDS patch of the grouped
+// evaluation kernel), coarsened until the brick key fits 16 bits (then
+// the patch no longer fits and the grouped kernel reads HBM only).
+BinParams bin_params(const GridData& g, int64_t* patch_volume) {
     BinParams bp{};
     bp.dim_count = g.dim;
+    bp.order = g.order;
     int B = 1;
     auto patch = [&](int b) {
-        double v = 8.0;
-        for (int d = 0; d < g.dim; ++d) v *= static_cast<double>(b + g.order);
+        int64_t v = 1;
+        for (int d = 0; d < g.dim; ++d) v *= b + g.order;
         return v;
     };
-    while (patch(2 * B) <= 16384.0) B *= 2;
+    while (patch(2 * B) * 8 <= 16384) B *= 2;
     for (;;) {
         int64_t nbin = 1;
         for (int d = 0; d < g.dim; ++d) nbin *= (g.setup[d].n + B - 1) / B;
         if (nbin <= 65536) break;
         B *= 2;
     }
+    int shift = 0;
+    while ((1 << shift) < B) ++shift;
     for (int d = 0; d < g.dim; ++d) {
-        const DimSetup& s = g.setup[d];
-        BinDim& b = bp.dim[d];
-        b.lo = s.lo;
-        b.period = s.hi - s.lo;
-        b.inv_period = 1.0 / b.period;
-        b.inv_dx = 1.0 / s.dx;
-        b.periodic = s.periodic ? 1 : 0;
-        b.ncell = static_cast<int32_t>(s.n);
-        int shift = 0;
-        while ((1 << shift) < B) ++shift;
-        b.brick_shift = shift;
-        b.nbrick = static_cast<int32_t>((s.n + B - 1) / B);
+        bp.dim[d] = dim_params(g, d);
+        bp.bricks.brick_shift[d] = shift;
+        bp.bricks.nbrick[d] = static_cast<int32_t>((g.setup[d].n + B - 1) / B);
     }
+    *patch_volume = (patch(B) * 8 <= 16384 && B + g.order <= 64) ? patch(B) : 0;
     return bp;
 }
 
@@ -732,6 +728,11 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
             single.push_back(o);
         }
     }
+    {
+        bool packed = os_out == 1 && os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;
+        for (int i = 0; i <= D; ++i) packed = packed && p.slot[i] == i;
+        p.out_packed = packed ? 1 : 0;
+    }
     hipError_t e = hipSuccess;
     const bool fast = uniform_path(g);
     TimedCall timed{g.device, {nullptr, nullptr, nullptr}};
@@ -756,7 +757,8 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     };
     if (fast && D >= 2 && npts >= kGroupMinPoints && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_UNGROUPED) &&
         bin_scan_tiles(npts) <= 4096) {
-        BinParams bp = bin_params(g);
+        int64_t patch_volume = 0;
+        BinParams bp = bin_params(g, &patch_volume);
         bp.coords = p.coords;
         bp.cs_dim = p.cs_dim;
         bp.cs_pt = p.cs_pt;
@@ -784,6 +786,9 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         p.cs_pt = sizeof(BinRecord) / sizeof(double);
         p.perm = reinterpret_cast<const uint32_t*>(rec_b) + offsetof(BinRecord, idx) / sizeof(uint32_t);
         p.perm_stride = sizeof(BinRecord) / sizeof(uint32_t);
+        p.key = reinterpret_cast<const uint32_t*>(rec_b) + offsetof(BinRecord, key) / sizeof(uint32_t);
+        p.bricks = bp.bricks;
+        p.patch_volume = static_cast<int32_t>(patch_volume);
     }
     if (timing && e == hipSuccess) e = hipEventRecord(timed.ev[1], stream);
     if (any_grad && e == hipSuccess) e = launch_eval_grad(D, g.order, fast, p, stream);

@@ -3,10 +3,13 @@
 // cache-resident patch of the table instead of gathering ~20 scattered
 // cache lines each from the whole table.
 //
-// Two counting-sort passes over a 16-bit brick key (low byte, then high
-// byte).  Each pass: per-block LDS histograms (no global atomics), a
+// A brick groups the points whose stencils start in the same B^D block of
+// control points (BrickGrid).  Two counting-sort passes over the 16-bit
+// brick key (low byte, then high byte).  Each pass: per-block LDS histograms (no global atomics), a
 // device-wide exclusive scan over the digit-major histogram, and a scatter
-// with LDS-atomic ranks.  Ranks inside a block are not stable, which only
+// with LDS-atomic ranks.  Chunks are spread over the XCDs in contiguous
+// ranges (xcd_chunk_index), so the runs that neighbouring chunks write for
+// one digit meet in the same L2.  Ranks inside a block are not stable, which only
 // blurs the grouping within one 2048-point block; the evaluation result
 // does not depend on the order.  Records carry the original coordinates
 // and index, so the evaluation reads them sequentially and writes each
@@ -14,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "span.hpp"
 
 namespace intp_amd {
 
@@ -25,17 +29,28 @@ constexpr int kChunk = kThreads * kPerThread;  // points per block
 constexpr int kRadix = 256;
 constexpr int kScanTile = 4096;  // entries per scan block
 
+// Brick of a point: per dimension, the first control point of its
+// stencil shifted down to the brick.  The span comes from the reference's
+// O(1) hint (Interpolation.hpp:277-295) without the exact-knot correction
+// of the evaluation: a point keyed one span off at a knot only leaves its
+// brick's patch (the evaluation then reads it from HBM); results never
+// depend on the key.
 __device__ __forceinline__ uint32_t brick_key(const BinParams& p, const double* x) {
     uint32_t key = 0;
 #pragma unroll
     for (int d = 0; d < kMaxDim; ++d) {
         if (d >= p.dim_count) break;
-        const BinDim& b = p.dim[d];
-        double r = x[d] - b.lo;
-        if (b.periodic && !(r >= 0.0 && r < b.period)) r -= floor(r * b.inv_period) * b.period;
-        double c = floor(r * b.inv_dx);
-        c = fmin(fmax(c, 0.0), static_cast<double>(b.ncell - 1));  // NaN -> 0
-        key = key * static_cast<uint32_t>(b.nbrick) + (static_cast<uint32_t>(c) >> b.brick_shift);
+        const DimParams& dp = p.dim[d];
+        double xd = x[d];
+        wrap_or_check(dp, xd, nullptr);
+        const int last = dp.nt - p.order - 2;
+        double v = (xd - dp.lo) * dp.inv_dx - dp.shift;
+        v = v > 0.0 ? v : 0.0;  // NaN -> 0
+        v = v < static_cast<double>(last) ? v : static_cast<double>(last);
+        int start = min(last, static_cast<int>(ceil(v)) + p.order) - p.order;
+        if (start >= dp.n) start -= static_cast<int>(dp.n);  // periodic, even order: the last span wraps
+        key = key * static_cast<uint32_t>(p.bricks.nbrick[d]) +
+              (static_cast<uint32_t>(start) >> p.bricks.brick_shift[d]);
     }
     return key;
 }
@@ -76,7 +91,8 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
     __shared__ uint32_t h[kRadix];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * kChunk;
+    const int64_t chunk = xcd_chunk_index();
+    const int64_t first = chunk * kChunk;
     for (int j = 0; j < kPerThread; ++j) {
         const int64_t i = first + j * kThreads + threadIdx.x;
         if (i >= p.npts) break;
@@ -91,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
         atomicAdd(&h[digit], 1u);
     }
     __syncthreads();
-    hist[static_cast<int64_t>(threadIdx.x) * p.nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[static_cast<int64_t>(threadIdx.x) * p.nblocks + chunk] = h[threadIdx.x];
 }
 
 // Scatter of one chunk: records are first placed in LDS in digit order
@@ -104,10 +120,11 @@ __global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, cons
     __shared__ uint32_t gbase[kRadix];
     __shared__ uint32_t lstart[kRadix];
     __shared__ BinRecord stage[kChunk];
+    const int64_t chunk = xcd_chunk_index();
     cnt[threadIdx.x] = 0;
-    gbase[threadIdx.x] = offs[static_cast<int64_t>(threadIdx.x) * p.nblocks + blockIdx.x];
+    gbase[threadIdx.x] = offs[static_cast<int64_t>(threadIdx.x) * p.nblocks + chunk];
     __syncthreads();
-    const int64_t first = static_cast<int64_t>(blockIdx.x) * kChunk;
+    const int64_t first = chunk * kChunk;
     BinRecord r[kPerThread];
     uint32_t digit[kPerThread], rank[kPerThread];
 #pragma unroll

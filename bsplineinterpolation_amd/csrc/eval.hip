@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "span.hpp"
 
 namespace intp_amd {
 
@@ -43,16 +44,6 @@ struct DimW {
 // ---------------------------------------------------------------------------
 // span search
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ void wrap_or_check(const DimParams& dp, double& x, int* bad) {
-    if (dp.periodic) {
-        double r = x - dp.lo;
-        if (!(r >= 0.0 && r < dp.period)) r = fmod(r, dp.period);
-        x = dp.lo + r + (x < dp.lo ? dp.period : 0.0);
-    } else if (bad != nullptr && (x < dp.lo || x > dp.hi)) {
-        *bad = 1;
-    }
-}
 
 template <int K>
 __device__ __forceinline__ int hint(const DimParams& dp, double x, int last) {
@@ -195,51 +186,21 @@ __device__ __forceinline__ void general_weights(const DimParams& dp, double x, i
 // basis weights, uniform grids
 // ---------------------------------------------------------------------------
 
-// Knot i of a uniform dimension without branches (selects only).
 template <int K>
 __device__ __forceinline__ double uknot_sel(const DimParams& dp, int i) {
-#pragma clang fp contract(off)
-    const double f = dp.xmin + (static_cast<double>(i) - dp.half) * dp.dx;
-    if (dp.periodic) return f;
-    return i <= K ? dp.xmin : (i >= dp.nt - K - 1 ? dp.xmax : f);
+    return uknot_sel(dp, K, i);
 }
 
-// Span of x on a uniform dimension: hint, one select-based correction,
-// then a check against the exact knots; a lane that still disagrees (NaN,
-// coordinates beyond int range) takes the binary search.
 template <int K>
 __device__ __forceinline__ int uniform_span(const DimParams& dp, double x) {
-    const int last = dp.nt - K - 2;
-    double v = (x - dp.lo) * dp.inv_dx - dp.shift;
-    v = v > 0.0 ? v : 0.0;
-    v = v < static_cast<double>(last) ? v : static_cast<double>(last);
-    int mu = min(last, static_cast<int>(ceil(v)) + K);
-    const double tl = uknot_sel<K>(dp, mu), th = uknot_sel<K>(dp, mu + 1);
-    mu += (x >= th && mu < last) ? 1 : ((x < tl && mu > K) ? -1 : 0);
-    const bool ok = (mu == K || uknot_sel<K>(dp, mu) <= x) && (mu == last || x < uknot_sel<K>(dp, mu + 1));
-    if (!ok) {
-        int lo = K + 1, hi = last + 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (x < uknot_sel<K>(dp, mid))
-                hi = mid;
-            else
-                lo = mid + 1;
-        }
-        mu = lo - 1;
-    }
-    return mu;
+    return uniform_span(dp, K, x);
 }
 
+// Weights from the span's basis polynomials P (row j: coefficients of
+// basis j in u = (x - t_mu) / dx, ascending powers).
 template <int K, bool GRAD>
-__device__ __forceinline__ void uniform_weights(const DimParams& dp, const double* __restrict__ poly, double x,
-                                                int r, int* bad, DimW<K, uint32_t>& dd) {
-    wrap_or_check(dp, x, bad);
-    const int last = dp.nt - K - 2;
-    const int mu = uniform_span<K>(dp, x);
-    const int cls = (mu - K < dp.nleft) ? 1 + (mu - K) : ((last - mu < dp.nright) ? 1 + dp.nleft + (last - mu) : 0);
-    const double u = (x - uknot_sel<K>(dp, mu)) * dp.inv_dx;
-    const double* __restrict__ P = poly + cls * (K + 1) * (K + 1);
+__device__ __forceinline__ void poly_weights(const double* __restrict__ P, double u, double inv_dx, int r,
+                                             DimW<K, uint32_t>& dd) {
     constexpr int W = K + 1;
     if constexpr (GRAD) {
 #pragma unroll
@@ -247,34 +208,54 @@ __device__ __forceinline__ void uniform_weights(const DimParams& dp, const doubl
             double v = P[j * W + K];
             double g = static_cast<double>(K) * P[j * W + K];
 #pragma unroll
-            for (int p = K - 1; p >= 0; --p) {
-                v = v * u + P[j * W + p];
-                if (p >= 1) g = g * u + static_cast<double>(p) * P[j * W + p];
+            for (int q = K - 1; q >= 0; --q) {
+                v = v * u + P[j * W + q];
+                if (q >= 1) g = g * u + static_cast<double>(q) * P[j * W + q];
             }
             dd.w[j] = v;
-            dd.dw[j] = K >= 1 ? g * dp.inv_dx : 0.0;
+            dd.dw[j] = K >= 1 ? g * inv_dx : 0.0;
         }
     } else {
-        // r-th derivative: sum_p p!/(p-r)! P_p u^(p-r), times dx^-r
+        // r-th derivative: sum_q q!/(q-r)! P_q u^(q-r), times dx^-r
         double scale = 1.0;
-        for (int i = 0; i < r; ++i) scale *= dp.inv_dx;
+        for (int i = 0; i < r; ++i) scale *= inv_dx;
 #pragma unroll
         for (int j = 0; j <= K; ++j) {
             double v = 0.0;
 #pragma unroll
-            for (int p = K; p >= 0; --p) {
-                if (p >= r) {
+            for (int q = K; q >= 0; --q) {
+                if (q >= r) {
                     double ff = 1.0;
 #pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (q < r) ff *= static_cast<double>(p - q);
-                    v = v * u + ff * P[j * W + p];
+                    for (int t = 0; t < K; ++t)
+                        if (t < r) ff *= static_cast<double>(q - t);
+                    v = v * u + ff * P[j * W + q];
                 }
             }
             dd.w[j] = r > K ? 0.0 : v * scale;
         }
     }
+}
+
+// Returns the span; fills the weights and the HBM offsets of the stencil.
+// `poly` is the LDS copy of the dimension's polynomial classes; when every
+// lane of the wave is on an interior span (class 0, the common case) the
+// coefficients are read with scalar loads from dp.poly instead (the same
+// values), which keeps them out of the LDS pipe.
+template <int K, bool GRAD>
+__device__ __forceinline__ int uniform_weights(const DimParams& dp, const double* __restrict__ poly, double x,
+                                               int r, int* bad, DimW<K, uint32_t>& dd) {
+    wrap_or_check(dp, x, bad);
+    const int last = dp.nt - K - 2;
+    const int mu = uniform_span<K>(dp, x);
+    const int cls = (mu - K < dp.nleft) ? 1 + (mu - K) : ((last - mu < dp.nright) ? 1 + dp.nleft + (last - mu) : 0);
+    const double u = (x - uknot_sel<K>(dp, mu)) * dp.inv_dx;
+    if (__all(cls == 0))
+        poly_weights<K, GRAD>(dp.poly, u, dp.inv_dx, r, dd);
+    else
+        poly_weights<K, GRAD>(poly + cls * (K + 1) * (K + 1), u, dp.inv_dx, r, dd);
     stencil_offsets<K>(dp, mu, dd.off);
+    return mu;
 }
 
 // ---------------------------------------------------------------------------
@@ -416,6 +397,21 @@ __device__ __forceinline__ double contract_single(const double* __restrict__ c, 
     return v;
 }
 
+// Stores the value and first partials of one point.
+template <int D>
+__device__ __forceinline__ void store_grad(const EvalParams& p, int64_t opt, const double (&res)[D + 1]) {
+    if (p.out_packed) {
+        double* o = p.out + opt * p.os_pt;
+#pragma unroll
+        for (int s = 0; s + 1 <= D; s += 2) *reinterpret_cast<double2*>(o + s) = make_double2(res[s], res[s + 1]);
+        if constexpr ((D + 1) % 2 == 1) o[D] = res[D];
+    } else {
+#pragma unroll
+        for (int s = 0; s <= D; ++s)
+            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -423,13 +419,7 @@ __device__ __forceinline__ double contract_single(const double* __restrict__ c, 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2).  Block b is remapped so that XCD x processes a contiguous range of
 // chunks: with grouped points, one L2 then holds each brick patch.
-__device__ __forceinline__ int64_t block_index() {
-    constexpr int64_t kXcd = 8;
-    int64_t b = blockIdx.x;
-    const int64_t per = static_cast<int64_t>(gridDim.x) / kXcd;
-    if (b < per * kXcd) b = (b % kXcd) * per + b / kXcd;
-    return b;
-}
+__device__ __forceinline__ int64_t block_index() { return xcd_chunk_index(); }
 
 template <int D, int K, bool GRAD, bool LASTC>
 __device__ __forceinline__ void general_point(const EvalParams& p, int64_t pt) {
@@ -475,9 +465,7 @@ __device__ __forceinline__ void uniform_point(const EvalParams& p, const double 
     if constexpr (GRAD) {
         double res[D + 1];
         contract_grad<D, K, LASTC>(p.ctrl, dd, res);
-#pragma unroll
-        for (int s = 0; s <= D; ++s)
-            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
+        store_grad<D>(p, opt, res);
     } else {
         const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
         p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : v;
@@ -499,6 +487,165 @@ __global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
     if (pt < p.npts) uniform_point<D, K, GRAD, LASTC>(p, s_poly, pt);
 }
 
+// ---- grouped evaluation with the brick patch in LDS ---------------------
+
+constexpr int kPatchMax = 2048;  // doubles of one brick patch (16 KB)
+
+// One point of a grouped chunk: if its stencil lies in the staged patch
+// (its brick is the chunk's), the stencil is read from LDS, otherwise from
+// HBM.  Same weights and contraction order either way.
+template <int D, int K, bool GRAD, bool LASTC>
+__device__ __forceinline__ void patch_point(const EvalParams& p,
+                                            const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
+                                            const double* s_patch, const int (&bstart)[D], const int (&bsize)[D],
+                                            const uint32_t (&lstride)[D], int64_t pt) {
+    const int64_t opt = static_cast<int64_t>(p.perm[pt * p.perm_stride]);
+    DimW<K, uint32_t> dd[D];
+    bool zero = false, inside = true;
+    int local[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int r = GRAD ? 1 : p.order[d];
+        zero = zero || r > K;
+        const int mu = uniform_weights<K, GRAD>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r,
+                                                p.bad, dd[d]);
+        const int n = static_cast<int>(p.dim[d].n);
+        int st = mu - K;  // first stencil point; periodic even orders: may be n (wraps to 0)
+        if (st >= n) st -= n;
+        int l = st - bstart[d];
+        if (p.dim[d].periodic && l < 0) l += n;
+        local[d] = l;
+        inside = inside && l >= 0 && l < bsize[d];
+    }
+    double res[D + 1];
+    if (inside) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int m = 0; m <= K; ++m) dd[d].off[m] = static_cast<uint32_t>(local[d] + m) * lstride[d];
+        if constexpr (GRAD)
+            contract_grad<D, K, true>(s_patch, dd, res);
+        else
+            res[0] = contract_single<D, K, true>(s_patch, dd);
+    } else {
+        if constexpr (GRAD)
+            contract_grad<D, K, LASTC>(p.ctrl, dd, res);
+        else
+            res[0] = contract_single<D, K, LASTC>(p.ctrl, dd);
+    }
+    if constexpr (GRAD) {
+        store_grad<D>(p, opt, res);
+    } else {
+        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : res[0];
+    }
+}
+
+// One workgroup per 256 grouped points: stages the brick of its middle
+// point, (B + K)^D control points (wrapped on periodic dimensions, zero
+// past the end of clamped ones, where no stencil reaches), then evaluates.
+template <int D, int K, bool GRAD, bool LASTC>
+__global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
+    constexpr int kClassMax = 1 + 2 * K;
+    constexpr int kCoef = (K + 1) * (K + 1);
+    __shared__ double s_poly[D][kClassMax * kCoef];
+    __shared__ double s_patch[kPatchMax];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int n = p.dim[d].nclass * kCoef;
+        for (int i = threadIdx.x; i < n; i += kBlock) s_poly[d][i] = p.dim[d].poly[i];
+    }
+    const int64_t first = block_index() * kBlock;
+    const int64_t pivot = min(first + kBlock / 2, p.npts - 1);
+    uint32_t rem = p.key[pivot * p.perm_stride];
+    int bstart[D], bsize[D], psize[D];
+    uint32_t lstride[D];
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        const uint32_t nb = static_cast<uint32_t>(p.bricks.nbrick[d]);
+        const uint32_t b = rem % nb;
+        rem /= nb;
+        bstart[d] = static_cast<int>(b << p.bricks.brick_shift[d]);
+        bsize[d] = 1 << p.bricks.brick_shift[d];
+        psize[d] = bsize[d] + K;
+    }
+    uint32_t vol = 1;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        lstride[d] = vol;
+        vol *= static_cast<uint32_t>(psize[d]);
+    }
+    if (vol > static_cast<uint32_t>(kPatchMax) || psize[D - 1] > (D == 3 ? 16 : 64) || (D == 3 && psize[1] > 16) ||
+        psize[0] > (D == 3 ? 16 : 64)) {
+        // not staged: every point reads HBM
+        vol = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) bsize[d] = 0;
+    }
+    // Each thread owns one position of the inner dimensions (16 x 16 for
+    // D = 3, 64 for D = 2, so their patch extents must fit) and walks
+    // dimension 0 with a running index: no divisions in the loop.
+    if (vol > 0) {
+        constexpr int kInner = D == 3 ? 16 : 64;     // threads along the last dimension
+        constexpr int kGroups = kBlock / (D == 3 ? 256 : 64);  // D = 2: 4 groups split dimension 0
+        const int t = threadIdx.x;
+        const int i_last = t % kInner;
+        const int i_mid = D == 3 ? t / kInner : 0;
+        const int group = D == 3 ? 0 : t / kInner;
+        bool ok = i_last < psize[D - 1] && (D == 2 || i_mid < psize[D == 3 ? 1 : 0]);
+        // wrapped element of a dimension, or -1 past the end of a clamped one
+        auto element = [&](int d, int i) -> int64_t {
+            int64_t g = bstart[d] + static_cast<int64_t>(i);
+            const int64_t n = p.dim[d].n;
+            if (g < n) return g;
+            if (!p.dim[d].periodic) return -1;
+            while (g >= n) g -= n;
+            return g;
+        };
+        uint32_t inner_off = 0;
+        uint32_t lds = 0;
+        if (ok) {
+            const int64_t gl = element(D - 1, i_last);
+            ok = gl >= 0;
+            inner_off = static_cast<uint32_t>(gl);
+            lds = static_cast<uint32_t>(i_last);
+            if constexpr (D == 3) {
+                const int64_t gm = element(1, i_mid);
+                ok = ok && gm >= 0;
+                inner_off += static_cast<uint32_t>(gm * p.dim[1].stride);
+                lds += static_cast<uint32_t>(i_mid) * lstride[1];
+            }
+        }
+        if (ok) {
+            // all loads first, then all LDS stores, so the loads overlap
+            constexpr int kRows = D == 3 ? 16 : 16;  // >= psize[0] / kGroups
+            const int n0 = static_cast<int>(p.dim[0].n);
+            const bool per0 = p.dim[0].periodic != 0;
+            const uint32_t stride0 = static_cast<uint32_t>(p.dim[0].stride);
+            double v[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const int i0 = group + r * kGroups;
+                v[r] = 0.0;
+                if (i0 < psize[0]) {
+                    int g0 = bstart[0] + i0;
+                    if (g0 >= n0) g0 = per0 ? g0 % n0 : -1;
+                    if (g0 >= 0) v[r] = p.ctrl[inner_off + static_cast<uint32_t>(g0) * stride0];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const int i0 = group + r * kGroups;
+                if (i0 < psize[0]) s_patch[lds + static_cast<uint32_t>(i0) * lstride[0]] = v[r];
+            }
+        }
+        // positions past the end of a clamped dimension stay unwritten: no
+        // stencil of a point inside the brick reaches them
+    }
+    __syncthreads();
+    const int64_t pt = first + threadIdx.x;
+    if (pt < p.npts) patch_point<D, K, GRAD, LASTC>(p, s_poly, s_patch, bstart, bsize, lstride, pt);
+}
+
 template <void (*KF)(EvalParams)>
 hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
     const int64_t blocks = (p.npts + kBlock - 1) / kBlock;
@@ -510,6 +657,11 @@ hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
 
 template <int D, int K, bool LASTC>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    if (uniform && p.patch_volume > 0) {
+        if (D < 2 || p.patch_volume > kPatchMax || !p.perm || !p.key) return hipErrorInvalidValue;
+        if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC>>(p, s);
+        return launch_chunks<eval_patch_kernel<D, K, false, LASTC>>(p, s);
+    }
     if (uniform) {
         if (grad) return launch_chunks<eval_uniform_kernel<D, K, true, LASTC>>(p, s);
         return launch_chunks<eval_uniform_kernel<D, K, false, LASTC>>(p, s);

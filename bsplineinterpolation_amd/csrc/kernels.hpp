@@ -31,6 +31,16 @@ struct DimParams {
     int32_t nleft, nright, nclass, pad2_;
 };
 
+// Bricks: along dimension d, the points whose stencil starts at control
+// point i (i = span - order) belong to brick i >> brick_shift[d]; the key
+// of a point is its brick index in C order.  Brick b's stencils read only
+// control points [b*B, b*B + B - 1 + order] (B = 1 << brick_shift), wrapped
+// on periodic dimensions.
+struct BrickGrid {
+    int32_t brick_shift[kMaxDim];
+    int32_t nbrick[kMaxDim];
+};
+
 struct EvalParams {
     DimParams dim[kMaxDim];
     const double* ctrl;
@@ -51,6 +61,16 @@ struct EvalParams {
     // perm[pt * perm_stride] (null: position pt)
     const uint32_t* perm;
     int64_t perm_stride;
+    // grouped evaluation with brick patches in LDS (patch_volume > 0):
+    // point pt's brick key is key[pt * perm_stride]; each workgroup
+    // stages one brick's (B + order)^D control points
+    const uint32_t* key;
+    BrickGrid bricks;
+    int32_t patch_volume;
+    // gradient kernels: the D + 1 outputs of a point are contiguous and
+    // 16-byte aligned (os_out == 1, slot[s] == s, os_pt even): written as
+    // 16-byte stores
+    int32_t out_packed;
 };
 
 // A query point carried through the grouping passes (32 bytes).
@@ -60,14 +80,11 @@ struct BinRecord {
     uint32_t key;  // brick id
 };
 
-struct BinDim {
-    double lo, period, inv_period, inv_dx;
-    int32_t periodic, ncell, brick_shift, nbrick;
-};
-
 struct BinParams {
-    BinDim dim[kMaxDim];
+    DimParams dim[kMaxDim];  // the grid (span search), uniform dimensions only
+    BrickGrid bricks;
     int32_t dim_count;
+    int32_t order;
     int32_t nblocks;
     const double* coords;
     int64_t cs_dim, cs_pt;
@@ -107,6 +124,20 @@ struct ShiftParams {
     int64_t n[kMaxDim];
     int64_t shift[kMaxDim];
 };
+
+#ifdef __HIPCC__
+// Workgroup b runs on XCD b % 8 (round-robin dispatch).  This maps the
+// workgroups of each XCD to one contiguous range of chunk indices, so
+// that chunks next to each other in memory are processed at the same
+// time by CUs sharing one L2.
+__device__ __forceinline__ int64_t xcd_chunk_index() {
+    constexpr int64_t kXcd = 8;
+    int64_t b = blockIdx.x;
+    const int64_t per = static_cast<int64_t>(gridDim.x) / kXcd;
+    if (b < per * kXcd) b = (b % kXcd) * per + b / kXcd;
+    return b;
+}
+#endif
 
 // Returns hipSuccess or the launch error; hipErrorNotSupported when no
 // kernel is compiled for (dim, order).  `uniform`: every dimension is
