@@ -99,24 +99,31 @@ class InterpolationFunction:
     ``f`` is a C-order array of values (numpy: host, torch: device).
     ``ranges[d]`` = (xmin, xmax) for a uniform dimension, or ``coords[d]`` =
     node coordinates for a non-uniform one (n[d] + periodic[d] of them).
+    ``precision``: "f64" (the reference's FP64 arithmetic) or
+    "f32_refined" (FP32 control points and evaluation, solved in FP32 with
+    ``sweeps`` rounds of FP64-residual refinement, intp_build_refined).
     """
 
     def __init__(self, order: int, f, periodic: Optional[Sequence[bool]] = None, ranges=None, coords=None,
-                 ctx: Optional[Context] = None, stream=None):
+                 ctx: Optional[Context] = None, stream=None, precision: str = "f64", sweeps: int = 2):
         lib = _lib.load()
         self.ctx = ctx or Context.default()
         desc, keep = _grid_desc(order, tuple(f.shape), periodic, ranges, coords)
         h = C.c_void_p()
+        refined = _refined(precision)
         if _is_torch(f):
             f = f.contiguous()
             assert str(f.dtype) == "torch.float64", "values must be float64"
             st = _stream_handle(stream) if stream is not None else _current_stream()
-            check(lib.intp_build(self.ctx.h, C.byref(desc), C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE,
-                                 C.byref(h), st))
+            vp, mem = C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE
         else:
             fa = np.ascontiguousarray(f, dtype=np.float64)
-            check(lib.intp_build(self.ctx.h, C.byref(desc), fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST,
-                                 C.byref(h), _stream_handle(stream)))
+            st = _stream_handle(stream)
+            vp, mem = fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST
+        if refined:
+            check(lib.intp_build_refined(self.ctx.h, C.byref(desc), vp, mem, int(sweeps), C.byref(h), st))
+        else:
+            check(lib.intp_build(self.ctx.h, C.byref(desc), vp, mem, C.byref(h), st))
         del keep
         self._set_handle(h)
 
@@ -134,6 +141,8 @@ class InterpolationFunction:
         self.dim = info.dim
         self.order = info.order
         self._info = info
+        self.precision = "f32_refined" if info.ctrl_bytes == 4 else "f64"
+        self.refine_sweeps = info.refine_sweeps if info.ctrl_bytes == 4 else None
 
     @classmethod
     def from_knots(cls, order: int, knots: Sequence[np.ndarray], ctrl: np.ndarray, periodic=None,
@@ -183,11 +192,13 @@ class InterpolationFunction:
         return out
 
     # ---- evaluation ---------------------------------------------------
-    def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None, timed: bool = False):
+    def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None, timed: bool = False,
+             out_dtype=None):
         """Batched evaluation.  ``points``: numpy [npts, D] (host) or a
         float64 torch tensor [D, npts] (SoA) / [npts, D] (AoS) on the
         device.  ``orders``: [nout, D] derivative orders (default: value).
-        Returns [nout, npts] (numpy or torch, like ``points``).  ``timed``
+        Returns [nout, npts] (numpy or torch, like ``points``), float64, or
+        float32 when ``out`` is float32 or ``out_dtype`` says so.  ``timed``
         records device events around the call (read with eval_timings())."""
         lib = _lib.load()
         D = self.dim
@@ -207,17 +218,24 @@ class InterpolationFunction:
             else:
                 npts, cs_dim, cs_pt = pts.shape[0], 1, D
             if out is None:
-                out = torch.empty((nout, npts), dtype=torch.float64, device=pts.device)
-            assert tuple(out.shape) == (nout, npts) and out.dtype == torch.float64 and out.is_cuda
+                dt = torch.float32 if out_dtype in (torch.float32, np.float32, "float32") else torch.float64
+                out = torch.empty((nout, npts), dtype=dt, device=pts.device)
+            assert tuple(out.shape) == (nout, npts) and out.dtype in (torch.float64, torch.float32) and out.is_cuda
+            if out.dtype == torch.float32:
+                flags |= _lib.INTP_EVAL_OUT_F32
             st = _stream_handle(stream) if stream is not None else _current_stream()
             check(lib.intp_eval(self.h, npts, C.cast(pts.data_ptr(), _dp), cs_dim, cs_pt, nout, op,
                                 C.cast(out.data_ptr(), _dp), out.stride(0), out.stride(1), flags, st))
             return out
         pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, D))
         npts = pts.shape[0]
-        res = np.empty((nout, npts))
-        check(lib.intp_eval(self.h, npts, pts.ctypes.data_as(_dp), 1, D, nout, op, res.ctypes.data_as(_dp),
-                            npts, 1, flags | _lib.INTP_EVAL_HOST_MEM, _stream_handle(stream)))
+        f32 = out_dtype in (np.float32, "float32")
+        res = np.empty((nout, npts), dtype=np.float32 if f32 else np.float64)
+        if f32:
+            flags |= _lib.INTP_EVAL_OUT_F32
+        check(lib.intp_eval(self.h, npts, pts.ctypes.data_as(_dp), 1, D, nout, op,
+                            C.cast(res.ctypes.data, _dp), npts, 1, flags | _lib.INTP_EVAL_HOST_MEM,
+                            _stream_handle(stream)))
         return res
 
     def __call__(self, *x):
@@ -246,6 +264,12 @@ class InterpolationFunction:
         pts = np.asarray(coord, dtype=np.float64).reshape(-1, self.dim)
         v = self.eval(pts, r, bounds=True)[0]
         return float(v[0]) if v.size == 1 else v
+
+
+def _refined(precision: str) -> bool:
+    if precision not in ("f64", "f32_refined"):
+        raise IntpError(_lib.INTP_E_INVALID, "precision must be 'f64' or 'f32_refined'")
+    return precision == "f32_refined"
 
 
 def eval_timings(capacity: int = 1 << 16):
@@ -282,7 +306,7 @@ class InterpolationFunctionTemplate:
             _lib._LIB.intp_template_destroy(self.h)
             self.h = None
 
-    def interpolate(self, f, stream=None) -> InterpolationFunction:
+    def interpolate(self, f, stream=None, precision: str = "f64", sweeps: int = 2) -> InterpolationFunction:
         lib = _lib.load()
         if tuple(f.shape) != self.shape:
             raise IntpError(_lib.INTP_E_INVALID, "value mesh shape does not match the template")
@@ -290,9 +314,13 @@ class InterpolationFunctionTemplate:
         if _is_torch(f):
             f = f.contiguous()
             st = _stream_handle(stream) if stream is not None else _current_stream()
-            check(lib.intp_interpolate(self.h, C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE, C.byref(h), st))
+            vp, mem = C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE
         else:
             fa = np.ascontiguousarray(f, dtype=np.float64)
-            check(lib.intp_interpolate(self.h, fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST, C.byref(h),
-                                       _stream_handle(stream)))
+            st = _stream_handle(stream)
+            vp, mem = fa.ctypes.data_as(_dp), _lib.INTP_MEM_HOST
+        if _refined(precision):
+            check(lib.intp_interpolate_refined(self.h, vp, mem, int(sweeps), C.byref(h), st))
+        else:
+            check(lib.intp_interpolate(self.h, vp, mem, C.byref(h), st))
         return InterpolationFunction._wrap(h, self.ctx)

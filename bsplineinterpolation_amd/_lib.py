@@ -27,6 +27,7 @@ INTP_EVAL_BOUNDS_CHECK = 1
 INTP_EVAL_HOST_MEM = 2
 INTP_EVAL_UNGROUPED = 4
 INTP_EVAL_TIMED = 8
+INTP_EVAL_OUT_F32 = 16
 
 # every symbol include/intp_amd.h declares
 EXPORTS = (
@@ -34,6 +35,7 @@ EXPORTS = (
     "intp_template_create", "intp_template_destroy", "intp_interpolate", "intp_build",
     "intp_spline_from_knots", "intp_spline_retain", "intp_spline_release", "intp_spline_get_info",
     "intp_spline_download_ctrl", "intp_spline_download_knots", "intp_eval", "intp_eval_timings",
+    "intp_interpolate_refined", "intp_build_refined",
 )
 
 
@@ -46,7 +48,8 @@ class GridDesc(C.Structure):
 class SplineInfo(C.Structure):
     _fields_ = [("dim", C.c_int32), ("order", C.c_int32), ("periodic", C.c_int32 * MAX_DIM),
                 ("uniform", C.c_int32 * MAX_DIM), ("n", C.c_int64 * MAX_DIM), ("nknots", C.c_int64 * MAX_DIM),
-                ("range_lo", C.c_double * MAX_DIM), ("range_hi", C.c_double * MAX_DIM)]
+                ("range_lo", C.c_double * MAX_DIM), ("range_hi", C.c_double * MAX_DIM),
+                ("ctrl_bytes", C.c_int32), ("refine_sweeps", C.c_int32)]
 
 
 _LIB = None
@@ -89,6 +92,8 @@ def load(path: str = LIB_PATH):
     lib.intp_eval.argtypes = [vp, C.c_int64, dp, C.c_int64, C.c_int64, C.c_int32, i32p, dp, C.c_int64,
                               C.c_int64, C.c_uint32, vp]
     lib.intp_eval_timings.argtypes = [dp, dp, C.c_int64, i64p]
+    lib.intp_interpolate_refined.argtypes = [vp, dp, C.c_int, C.c_int32, C.POINTER(vp), vp]
+    lib.intp_build_refined.argtypes = [vp, C.POINTER(GridDesc), dp, C.c_int, C.c_int32, C.POINTER(vp), vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

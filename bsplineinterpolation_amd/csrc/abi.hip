@@ -7,9 +7,11 @@
 #include <cstddef>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -18,6 +20,30 @@
 #include "setup.hpp"
 
 using namespace intp_amd;
+
+namespace intp_amd {
+
+hipError_t launch_eval(int dim, int order, bool grad, bool uniform, bool f32, const EvalParams& p, hipStream_t s) {
+    switch (dim * 2 + (f32 ? 1 : 0)) {
+        case 2: return eval_dispatch_f64_d1(order, grad, uniform, p, s);
+        case 3: return eval_dispatch_f32_d1(order, grad, uniform, p, s);
+        case 4: return eval_dispatch_f64_d2(order, grad, uniform, p, s);
+        case 5: return eval_dispatch_f32_d2(order, grad, uniform, p, s);
+        case 6: return eval_dispatch_f64_d3(order, grad, uniform, p, s);
+        case 7: return eval_dispatch_f32_d3(order, grad, uniform, p, s);
+        default: return hipErrorNotSupported;
+    }
+}
+
+hipError_t launch_eval_grad(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s) {
+    return launch_eval(dim, order, true, uniform, f32, p, s);
+}
+
+hipError_t launch_eval_single(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s) {
+    return launch_eval(dim, order, false, uniform, f32, p, s);
+}
+
+}  // namespace intp_amd
 
 namespace {
 
@@ -187,10 +213,16 @@ struct GridData {
     const double* bottom[kMaxDim] = {};
     const double* right[kMaxDim] = {};
     const double* poly[kMaxDim] = {};
+    const double* colloc[kMaxDim] = {};  // unfactored collocation rows (refined builds)
+    const float* lower32[kMaxDim] = {};   // LU factors rounded to FP32
+    const float* upper32[kMaxDim] = {};
+    const float* diag32[kMaxDim] = {};
+    const float* bottom32[kMaxDim] = {};
+    const float* right32[kMaxDim] = {};
     bool has_factors = false;
     int64_t total = 1;
-    std::vector<double> image;  // host copy of `dev`, the source of its upload
-    double* dev = nullptr;
+    std::vector<unsigned char> image;  // host copy of `dev`, the source of its upload
+    unsigned char* dev = nullptr;
     hipEvent_t uploaded = nullptr;
     UseTracker uses;
 
@@ -209,39 +241,50 @@ struct GridData {
 // uploads it with one asynchronous copy on `stream`.
 int upload_grid(GridData& g, hipStream_t stream) {
     struct Item {
-        const std::vector<double>* src;
-        const double** view;
+        const void* src;
+        size_t bytes;
+        std::function<void(const unsigned char*)> set_view;
     };
     std::vector<Item> items;
+    auto add = [&](const auto& v, auto& view) {
+        using P = std::remove_reference_t<decltype(view)>;
+        items.push_back({v.data(), v.size() * sizeof(v[0]),
+                         [&view](const unsigned char* at) { view = reinterpret_cast<P>(at); }});
+    };
     for (int d = 0; d < g.dim; ++d) {
         DimSetup& s = g.setup[d];
-        items.push_back({&s.knots, &g.knots[d]});
-        items.push_back({&s.recip, &g.recip[d]});
-        items.push_back({&s.poly, &g.poly[d]});
+        add(s.knots, g.knots[d]);
+        add(s.recip, g.recip[d]);
+        add(s.poly, g.poly[d]);
         if (g.has_factors) {
-            items.push_back({&s.lu.lower, &g.lower[d]});
-            items.push_back({&s.lu.upper, &g.upper[d]});
-            items.push_back({&s.lu.diag, &g.diag[d]});
-            items.push_back({&s.lu.bottom, &g.bottom[d]});
-            items.push_back({&s.lu.right, &g.right[d]});
+            add(s.lu.lower, g.lower[d]);
+            add(s.lu.upper, g.upper[d]);
+            add(s.lu.diag, g.diag[d]);
+            add(s.lu.bottom, g.bottom[d]);
+            add(s.lu.right, g.right[d]);
+            add(s.colloc, g.colloc[d]);
+            add(s.lower32, g.lower32[d]);
+            add(s.upper32, g.upper32[d]);
+            add(s.diag32, g.diag32[d]);
+            add(s.bottom32, g.bottom32[d]);
+            add(s.right32, g.right32[d]);
         }
     }
     std::vector<size_t> offset;
     size_t total = 0;
     for (const Item& it : items) {
         offset.push_back(total);
-        total += (it.src->size() + 31) / 32 * 32;
+        total += (it.bytes + 255) / 256 * 256;
     }
-    g.image.assign(total, 0.0);
+    g.image.assign(total, 0);
     for (size_t i = 0; i < items.size(); ++i)
-        std::copy(items[i].src->begin(), items[i].src->end(), g.image.begin() + offset[i]);
-    const size_t bytes = total * sizeof(double);
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&g.dev), bytes, stream), "hipMallocAsync (grid tables)");
-    HIP_TRY(hipMemcpyAsync(g.dev, g.image.data(), bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+        if (items[i].bytes) std::memcpy(g.image.data() + offset[i], items[i].src, items[i].bytes);
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&g.dev), total, stream), "hipMallocAsync (grid tables)");
+    HIP_TRY(hipMemcpyAsync(g.dev, g.image.data(), total, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
     HIP_TRY(g.uses.record(stream), "hipEventRecord");
     HIP_TRY(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming), "hipEventCreate");
     HIP_TRY(hipEventRecord(g.uploaded, stream), "hipEventRecord");
-    for (size_t i = 0; i < items.size(); ++i) *items[i].view = items[i].src->empty() ? nullptr : g.dev + offset[i];
+    for (size_t i = 0; i < items.size(); ++i) items[i].set_view(items[i].bytes ? g.dev + offset[i] : nullptr);
     return INTP_OK;
 }
 
@@ -278,8 +321,10 @@ struct intp_template_s {
 struct intp_spline_s {
     std::atomic<int> refs{1};
     std::shared_ptr<GridData> grid;
-    double* ctrl = nullptr;
-    UseTracker uses;  // streams that read `ctrl` (solve and every eval)
+    void* ctrl = nullptr;  // FP64, or FP32 for a refined build
+    int ctrl_bytes = 8;
+    int sweeps = -1;       // refinement sweeps of a refined build
+    UseTracker uses;       // streams that read `ctrl` (solve and every eval)
     ~intp_spline_s() {
         // ordered on the release stream before the grid's own release,
         // which runs when `grid` drops its last reference below
@@ -329,54 +374,73 @@ int make_grid(intp_ctx ctx, const intp_grid_desc* desc, hipStream_t stream, bool
     return INTP_OK;
 }
 
-// Control-point solve of `values` (device) into `ctrl` (device): one
-// line-solve pass per axis, last axis first (InterpolationTemplate.hpp:448-580).
-// The first pass reads `values` with the periodic shift of
-// InterpolationTemplate.hpp:452-462 applied; when its lines do not fit
-// the LDS-tiled kernel, a shift copy precedes an in-place first pass.
-int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_t stream) {
+// Periodic index shift of the right-hand sides along each dimension
+// (InterpolationTemplate.hpp:452-462).
+void solve_shifts(const GridData& g, int64_t (&shift)[kMaxDim]) {
+    for (int d = 0; d < kMaxDim; ++d) shift[d] = 0;
+    for (int d = 0; d < g.dim; ++d) shift[d] = g.setup[d].periodic ? (g.order / 2) % g.setup[d].n : 0;
+}
+
+// Element types of the solve along axis d: FP64 builds are FP64 through;
+// refined builds store FP32 and compute in FP32 with FP32 factors, except
+// periodic dimensions of order >= 5, whose factors stay FP64.
+SolveTypes axis_types(const GridData& g, bool refined, int d) {
+    if (!refined) return SolveTypes::kF64;
+    return (g.setup[d].periodic && g.order >= 5) ? SolveTypes::kF32StoreF64Compute : SolveTypes::kF32;
+}
+
+int compute_bytes(SolveTypes t) { return t == SolveTypes::kF32 ? 4 : 8; }
+
+// Line solves along every axis, last axis first (InterpolationTemplate.hpp:448-580),
+// on table x (FP64, or FP32 when `refined`).  With `values` set, the first
+// pass reads them (FP64) with the periodic shift applied; when its lines do
+// not fit the LDS-tiled kernel, a shift copy precedes an in-place first
+// pass.  Without `values`, x already holds shifted right-hand sides.  With
+// `accum` set, the last pass adds its result into accum instead of x.
+int solve_axes(const GridData& g, bool refined, const double* values, void* x, void* accum, hipStream_t stream) {
     const int last = g.dim - 1;
-    int64_t shift[kMaxDim] = {};
+    int64_t shift[kMaxDim];
+    solve_shifts(g, shift);
     bool any_shift = false;
-    for (int d = 0; d < g.dim; ++d) {
-        shift[d] = g.setup[d].periodic ? (g.order / 2) % g.setup[d].n : 0;
-        any_shift = any_shift || shift[d] != 0;
-    }
-    const bool fused = solve_tile_lines(g.setup[last].n) > 0;
-    if (!fused) {
-        if (any_shift) {
+    for (int d = 0; d < g.dim; ++d) any_shift = any_shift || shift[d] != 0;
+    const size_t elem = refined ? sizeof(float) : sizeof(double);
+    const bool fused =
+        values != nullptr && solve_tile_lines(g.setup[last].n, compute_bytes(axis_types(g, refined, last))) > 0;
+    if (values != nullptr && !fused) {
+        if (any_shift || refined) {
             ShiftParams sh{};
             sh.src = values;
-            sh.dst = ctrl;
+            sh.dst = x;
             sh.total = g.total;
             sh.dim = g.dim;
             for (int d = 0; d < g.dim; ++d) {
                 sh.n[d] = g.setup[d].n;
                 sh.shift[d] = shift[d];
             }
-            HIP_TRY(launch_shift_copy(sh, stream), "shift copy launch");
-        } else if (values != ctrl) {
-            HIP_TRY(hipMemcpyAsync(ctrl, values, static_cast<size_t>(g.total) * sizeof(double),
-                                   hipMemcpyDeviceToDevice, stream),
+            HIP_TRY(launch_shift_copy(sh, refined, stream), "shift copy launch");
+        } else if (values != x) {
+            HIP_TRY(hipMemcpyAsync(x, values, static_cast<size_t>(g.total) * elem, hipMemcpyDeviceToDevice, stream),
                     "hipMemcpyAsync");
         }
     }
     for (int a = last; a >= 0; --a) {
         const DimSetup& s = g.setup[a];
+        const SolveTypes types = axis_types(g, refined, a);
+        const bool f32 = types == SolveTypes::kF32;
         SolveParams sp{};
-        sp.x = ctrl;
+        sp.x = x;
         sp.n = s.n;
         int64_t inner = 1, outer = 1;
         for (int d = a + 1; d < g.dim; ++d) inner *= g.setup[d].n;
         for (int d = 0; d < a; ++d) outer *= g.setup[d].n;
         sp.inner = inner;
         sp.nlines = inner * outer;
-        sp.lower = g.lower[a];
-        sp.upper = g.upper[a];
-        sp.diag = g.diag[a];
-        sp.bottom = g.bottom[a];
-        sp.right = g.right[a];
-        sp.lines_per_block = solve_tile_lines(s.n);
+        sp.lower = f32 ? static_cast<const void*>(g.lower32[a]) : g.lower[a];
+        sp.upper = f32 ? static_cast<const void*>(g.upper32[a]) : g.upper[a];
+        sp.diag = f32 ? static_cast<const void*>(g.diag32[a]) : g.diag[a];
+        sp.bottom = f32 ? static_cast<const void*>(g.bottom32[a]) : g.bottom[a];
+        sp.right = f32 ? static_cast<const void*>(g.right32[a]) : g.right[a];
+        sp.lines_per_block = solve_tile_lines(s.n, compute_bytes(types));
         if (a == last && fused) {
             sp.src = values;
             sp.outer_dims = last;
@@ -386,9 +450,72 @@ int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_
             }
             sp.shift_last = shift[last];
         }
-        hipError_t e = launch_solve(s.lu.p, s.lu.cyclic, sp, stream);
+        const bool add_after = a == 0 && accum != nullptr && sp.lines_per_block == 0;
+        if (a == 0 && accum != nullptr && !add_after) sp.accum = accum;
+        hipError_t e = launch_solve(s.lu.p, s.lu.cyclic, types, sp, stream);
         if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
         HIP_TRY(e, "solve launch");
+        if (add_after) {
+            if (!refined) return fail(INTP_E_INVALID, "accumulation is defined for FP32 tables only");
+            HIP_TRY(launch_add_f32(static_cast<float*>(accum), static_cast<const float*>(x), g.total, stream),
+                    "correction launch");
+        }
+    }
+    return INTP_OK;
+}
+
+// FP64 control-point solve of `values` into `ctrl` (both device).
+int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_t stream) {
+    return solve_axes(g, false, values, ctrl, nullptr, stream);
+}
+
+// Refined-FP32 control points of `values` into `ctrl` (FP32): an FP32
+// solve, then `sweeps` rounds of an FP64 residual r = S f - (M_0 (x) ...) c
+// and an FP32 correction solve added to c.  `scratch` holds N floats and
+// 2 N doubles (N doubles for 1D).
+int refine_into(const GridData& g, const double* values, float* ctrl, int sweeps, unsigned char* scratch,
+                hipStream_t stream) {
+    int rc = solve_axes(g, true, values, ctrl, nullptr, stream);
+    if (rc != INTP_OK) return rc;
+    float* delta = reinterpret_cast<float*>(scratch);
+    double* t[2] = {reinterpret_cast<double*>(scratch + ((g.total * 4 + 255) / 256) * 256), nullptr};
+    t[1] = t[0] + ((g.total + 31) / 32) * 32;
+    int64_t shift[kMaxDim];
+    solve_shifts(g, shift);
+    for (int sweep = 0; sweep < sweeps; ++sweep) {
+        // residual, axes last to first: c -> t -> ... -> delta
+        const void* in = ctrl;
+        bool in_f32 = true;
+        for (int a = g.dim - 1; a >= 0; --a) {
+            MatvecParams mp{};
+            const DimSetup& s = g.setup[a];
+            mp.x = in;
+            mp.colloc = g.colloc[a];
+            mp.w = s.colloc_w;
+            mp.periodic = s.periodic ? 1 : 0;
+            mp.n = s.n;
+            int64_t inner = 1;
+            for (int d = a + 1; d < g.dim; ++d) inner *= g.setup[d].n;
+            mp.inner = inner;
+            mp.total = g.total;
+            const bool final_pass = a == 0;
+            if (final_pass) {
+                mp.y = delta;
+                mp.src = values;
+                mp.dim = g.dim;
+                for (int d = 0; d < g.dim; ++d) {
+                    mp.dn[d] = g.setup[d].n;
+                    mp.dshift[d] = shift[d];
+                }
+            } else {
+                mp.y = t[a % 2];
+            }
+            HIP_TRY(launch_band_matvec(mp, in_f32, final_pass, stream), "residual launch");
+            in = mp.y;
+            in_f32 = false;
+        }
+        // correction: solve on delta (already shifted), added into ctrl
+        if ((rc = solve_axes(g, true, nullptr, delta, ctrl, stream)) != INTP_OK) return rc;
     }
     return INTP_OK;
 }
@@ -425,10 +552,11 @@ DimParams dim_params(const GridData& g, int d) {
 constexpr int64_t kGroupMinPoints = int64_t{1} << 20;
 
 // Bricks of B^D stencil starts, B a power of two: the largest B whose
-// (B + order)^D patch stays within 16 KB (the LDS patch of the grouped
+// (B + order)^D patch of table elements (elem_bytes each) stays within
+// 16 KB (the LDS patch of the grouped
 // evaluation kernel), coarsened until the brick key fits 16 bits (then
 // the patch no longer fits and the grouped kernel reads HBM only).
-BinParams bin_params(const GridData& g, int64_t* patch_volume) {
+BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
     BinParams bp{};
     bp.dim_count = g.dim;
     bp.order = g.order;
@@ -438,7 +566,7 @@ BinParams bin_params(const GridData& g, int64_t* patch_volume) {
         for (int d = 0; d < g.dim; ++d) v *= b + g.order;
         return v;
     };
-    while (patch(2 * B) * 8 <= 16384) B *= 2;
+    while (patch(2 * B) * elem_bytes <= 16384) B *= 2;
     for (;;) {
         int64_t nbin = 1;
         for (int d = 0; d < g.dim; ++d) nbin *= (g.setup[d].n + B - 1) / B;
@@ -452,7 +580,7 @@ BinParams bin_params(const GridData& g, int64_t* patch_volume) {
         bp.bricks.brick_shift[d] = shift;
         bp.bricks.nbrick[d] = static_cast<int32_t>((g.setup[d].n + B - 1) / B);
     }
-    *patch_volume = (patch(B) * 8 <= 16384 && B + g.order <= 64) ? patch(B) : 0;
+    *patch_volume = (patch(B) * elem_bytes <= 16384 && B + g.order <= 64) ? patch(B) : 0;
     return bp;
 }
 
@@ -511,43 +639,89 @@ int intp_template_destroy(intp_template t) {
     return INTP_OK;
 }
 
-int intp_interpolate(intp_template t, const double* values, int values_mem, intp_spline* out, void* stream_) {
+}  // extern "C"
+
+namespace {
+
+constexpr int kMaxRefineSweeps = 3;
+
+// Control points of `values` on a template's grid: FP64, or FP32 refined
+// with `sweeps` FP64-residual corrections.
+int interpolate_impl(intp_template t, const double* values, int values_mem, bool refined, int sweeps,
+                     intp_spline* out, hipStream_t stream) {
     if (!t || !values || !out) return fail(INTP_E_INVALID, "null argument");
+    if (values_mem != INTP_MEM_HOST && values_mem != INTP_MEM_DEVICE)
+        return fail(INTP_E_INVALID, "values_mem must be INTP_MEM_HOST or INTP_MEM_DEVICE");
+    if (refined && (sweeps < 0 || sweeps > kMaxRefineSweeps))
+        return fail(INTP_E_INVALID, "refinement sweeps must be 0.." + std::to_string(kMaxRefineSweeps));
     const GridData& g = *t->grid;
-    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    if (refined && g.total >= (int64_t{1} << 32))
+        return fail(INTP_E_UNSUPPORTED, "refined builds need fewer than 2^32 control points");
     DeviceScope scope(g.device);
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
-    const size_t bytes = static_cast<size_t>(g.total) * sizeof(double);
+    const size_t values_bytes = static_cast<size_t>(g.total) * sizeof(double);
     std::unique_ptr<intp_spline_s> s(new intp_spline_s());
     s->grid = t->grid;
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&s->ctrl), bytes, stream), "hipMallocAsync (control points)");
+    s->ctrl_bytes = refined ? 4 : 8;
+    s->sweeps = refined ? sweeps : -1;
+    HIP_TRY(hipMallocAsync(&s->ctrl, static_cast<size_t>(g.total) * s->ctrl_bytes, stream),
+            "hipMallocAsync (control points)");
     const double* src = values;
     double* staged = nullptr;
+    unsigned char* scratch = nullptr;
+    int rc = INTP_OK;
+    hipError_t e = hipSuccess;
     if (values_mem == INTP_MEM_HOST) {
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&staged), bytes, stream), "hipMallocAsync");
-        HIP_TRY(hipMemcpyAsync(staged, values, bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
+        e = hipMallocAsync(reinterpret_cast<void**>(&staged), values_bytes, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(staged, values, values_bytes, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) rc = hip_fail(e, "staging values");
         src = staged;
-    } else if (values_mem != INTP_MEM_DEVICE) {
-        return fail(INTP_E_INVALID, "values_mem must be INTP_MEM_HOST or INTP_MEM_DEVICE");
     }
-    int rc = solve_into(g, src, s->ctrl, stream);
+    if (rc == INTP_OK && refined && sweeps > 0) {
+        const size_t bytes = ((static_cast<size_t>(g.total) * 4 + 255) / 256) * 256 +
+                             2 * ((static_cast<size_t>(g.total) + 31) / 32) * 32 * sizeof(double);
+        e = hipMallocAsync(reinterpret_cast<void**>(&scratch), bytes, stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipMallocAsync (refinement scratch)");
+    }
+    if (rc == INTP_OK)
+        rc = refined ? refine_into(g, src, static_cast<float*>(s->ctrl), sweeps, scratch, stream)
+                     : solve_into(g, src, static_cast<double*>(s->ctrl), stream);
     // the spline's release waits for the solve; the grid's for its use here
     const hipError_t er = s->uses.record(stream);
     const hipError_t eg = t->grid->uses.record(stream);
     if (rc == INTP_OK && er != hipSuccess) rc = hip_fail(er, "hipEventRecord");
     if (rc == INTP_OK && eg != hipSuccess) rc = hip_fail(eg, "hipEventRecord");
+    if (scratch) (void)hipFreeAsync(scratch, stream);
     if (staged) {
-        if (rc == INTP_OK) {
-            HIP_TRY(hipFreeAsync(staged, stream), "hipFreeAsync");
-            HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        } else {
-            (void)hipFreeAsync(staged, stream);
-            (void)hipStreamSynchronize(stream);
-        }
+        (void)hipFreeAsync(staged, stream);
+        const hipError_t es = hipStreamSynchronize(stream);  // the caller's host buffer must outlive the copy
+        if (rc == INTP_OK && es != hipSuccess) rc = hip_fail(es, "hipStreamSynchronize");
     }
     if (rc != INTP_OK) return rc;
     *out = s.release();
     return INTP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int intp_interpolate(intp_template t, const double* values, int values_mem, intp_spline* out, void* stream) {
+    return interpolate_impl(t, values, values_mem, false, 0, out, static_cast<hipStream_t>(stream));
+}
+
+int intp_interpolate_refined(intp_template t, const double* values, int values_mem, int32_t sweeps,
+                             intp_spline* out, void* stream) {
+    return interpolate_impl(t, values, values_mem, true, sweeps, out, static_cast<hipStream_t>(stream));
+}
+
+int intp_build_refined(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem,
+                       int32_t sweeps, intp_spline* out, void* stream) {
+    if (!out) return fail(INTP_E_INVALID, "null output handle");
+    intp_template_s t;
+    const int rc = make_grid(ctx, desc, static_cast<hipStream_t>(stream), false, t.grid);
+    if (rc != INTP_OK) return rc;
+    return intp_interpolate_refined(&t, values, values_mem, sweeps, out, stream);
 }
 
 int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem, intp_spline* out,
@@ -618,6 +792,8 @@ int intp_spline_get_info(intp_spline s, intp_spline_info* out) {
         out->range_lo[d] = g.setup[d].lo;
         out->range_hi[d] = g.setup[d].hi;
     }
+    out->ctrl_bytes = s->ctrl_bytes;
+    out->refine_sweeps = s->sweeps;
     return INTP_OK;
 }
 
@@ -628,6 +804,15 @@ int intp_spline_download_ctrl(intp_spline s, double* host, int64_t count) {
     DeviceRuntime* rt = device_runtime(s->grid->device);
     if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
     s->uses.order_before(rt->setup);
+    if (s->ctrl_bytes == 4) {
+        std::vector<float> tmp(static_cast<size_t>(count));
+        HIP_TRY(hipMemcpyAsync(tmp.data(), s->ctrl, static_cast<size_t>(count) * sizeof(float),
+                               hipMemcpyDeviceToHost, rt->setup),
+                "hipMemcpyAsync");
+        HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
+        for (int64_t i = 0; i < count; ++i) host[i] = static_cast<double>(tmp[static_cast<size_t>(i)]);
+        return INTP_OK;
+    }
     HIP_TRY(hipMemcpyAsync(host, s->ctrl, static_cast<size_t>(count) * sizeof(double), hipMemcpyDeviceToHost,
                            rt->setup),
             "hipMemcpyAsync");
@@ -669,10 +854,12 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     p.npts = npts;
 
     const bool host = (flags & INTP_EVAL_HOST_MEM) != 0;
+    const bool out_f32 = (flags & INTP_EVAL_OUT_F32) != 0;
+    const size_t out_elem = out_f32 ? sizeof(float) : sizeof(double);
     const size_t cext = static_cast<size_t>((D - 1) * cs_dim + (npts - 1) * cs_pt + 1);
     const size_t oext = static_cast<size_t>((nout - 1) * os_out + (npts - 1) * os_pt + 1);
     double* dcoords = nullptr;
-    double* dout = nullptr;
+    void* dout = nullptr;
     int* dbad = nullptr;
     auto cleanup = [&]() {
         if (dcoords) (void)hipFreeAsync(dcoords, stream);
@@ -681,19 +868,22 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     };
     if (host) {
         hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&dcoords), cext * sizeof(double), stream);
-        if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&dout), oext * sizeof(double), stream);
+        if (e == hipSuccess) e = hipMallocAsync(&dout, oext * out_elem, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dcoords, coords, cext * sizeof(double), hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(dout, out, oext * sizeof(double), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dout, out, oext * out_elem, hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) {
             cleanup();
             return hip_fail(e, "staging host buffers");
         }
         p.coords = dcoords;
-        p.out = dout;
     } else {
         p.coords = coords;
-        p.out = out;
     }
+    void* const out_dev = host ? dout : static_cast<void*>(out);
+    if (out_f32)
+        p.out32 = static_cast<float*>(out_dev);
+    else
+        p.out = static_cast<double*>(out_dev);
     if (flags & INTP_EVAL_BOUNDS_CHECK) {
         hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&dbad), sizeof(int), stream);
         if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), stream);
@@ -729,7 +919,7 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         }
     }
     {
-        bool packed = os_out == 1 && os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;
+        bool packed = !out_f32 && os_out == 1 && os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;
         for (int i = 0; i <= D; ++i) packed = packed && p.slot[i] == i;
         p.out_packed = packed ? 1 : 0;
     }
@@ -758,7 +948,7 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     if (fast && D >= 2 && npts >= kGroupMinPoints && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_UNGROUPED) &&
         bin_scan_tiles(npts) <= 4096) {
         int64_t patch_volume = 0;
-        BinParams bp = bin_params(g, &patch_volume);
+        BinParams bp = bin_params(g, s->ctrl_bytes, &patch_volume);
         bp.coords = p.coords;
         bp.cs_dim = p.cs_dim;
         bp.cs_pt = p.cs_pt;
@@ -791,11 +981,12 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         p.patch_volume = static_cast<int32_t>(patch_volume);
     }
     if (timing && e == hipSuccess) e = hipEventRecord(timed.ev[1], stream);
-    if (any_grad && e == hipSuccess) e = launch_eval_grad(D, g.order, fast, p, stream);
+    const bool f32 = s->ctrl_bytes == 4;
+    if (any_grad && e == hipSuccess) e = launch_eval_grad(D, g.order, fast, f32, p, stream);
     for (size_t i = 0; e == hipSuccess && i < single.size(); ++i) {
         for (int d = 0; d < D; ++d) p.order[d] = orders[single[i] * D + d];
         p.out_index = single[i];
-        e = launch_eval_single(D, g.order, fast, p, stream);
+        e = launch_eval_single(D, g.order, fast, f32, p, stream);
     }
     if (timing) {
         if (e == hipSuccess) e = hipEventRecord(timed.ev[2], stream);
@@ -826,7 +1017,7 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         }
     }
     if (host) {
-        e = hipMemcpyAsync(out, dout, oext * sizeof(double), hipMemcpyDeviceToHost, stream);
+        e = hipMemcpyAsync(out, dout, oext * out_elem, hipMemcpyDeviceToHost, stream);
         if (e != hipSuccess) {
             cleanup();
             return hip_fail(e, "output copy");

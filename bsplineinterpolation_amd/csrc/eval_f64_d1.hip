@@ -1,0 +1,10 @@
+// eval_f64_d1.hip -- evaluation kernels for 1D splines with double tables.
+#include "eval_impl.hpp"
+
+namespace intp_amd {
+
+hipError_t eval_dispatch_f64_d1(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    return eval_dispatch<1, double>(order, grad, uniform, p, s);
+}
+
+}  // namespace intp_amd

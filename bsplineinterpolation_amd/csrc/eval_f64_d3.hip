@@ -1,0 +1,10 @@
+// eval_f64_d3.hip -- evaluation kernels for 3D splines with double tables.
+#include "eval_impl.hpp"
+
+namespace intp_amd {
+
+hipError_t eval_dispatch_f64_d3(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    return eval_dispatch<3, double>(order, grad, uniform, p, s);
+}
+
+}  // namespace intp_amd

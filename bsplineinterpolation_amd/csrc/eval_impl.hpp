@@ -1,4 +1,6 @@
-// eval.hip -- batched evaluation of tensor-product B-splines on gfx950.
+// eval_impl.hpp -- batched evaluation of tensor-product B-splines on
+// gfx950 (kernel templates; eval_f64_d*.hip / eval_f32_d*.hip instantiate
+// them per table type and dimension).
 //
 // One lane per query point.  Per dimension, in registers: periodic wrap
 // (BSpline.hpp:130-136) and the knot span from the uniform-grid hint with a
@@ -17,6 +19,8 @@
 // differentiated), so all requested outputs share one gather of the
 // (k+1)^D stencil.  The stencil is contracted one axis at a time, last
 // (contiguous) axis first.
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,11 +38,11 @@ constexpr int kBlock = 256;
 #define INTP_OUTER_UNROLL_MAX_K 2  // fully unroll the outer stencil loop up to this order
 #endif
 
-template <int K, typename Off>
+template <int K, typename Off, typename W>
 struct DimW {
-    double w[K + 1];   // value weights (or the single output's weights)
-    double dw[K + 1];  // first-derivative weights (gradient kernels)
-    Off off[K + 1];    // element offsets of the K+1 stencil planes
+    W w[K + 1];      // value weights (or the single output's weights)
+    W dw[K + 1];     // first-derivative weights (gradient kernels)
+    Off off[K + 1];  // element offsets of the K+1 stencil planes
 };
 
 // ---------------------------------------------------------------------------
@@ -151,9 +155,9 @@ __device__ __forceinline__ void stencil_offsets(const DimParams& dp, int mu, Off
 
 // General grids: weights of the value (w) and, if `grad`, of d/dx (dw);
 // otherwise w holds the r-th derivative.
-template <int K, bool GRAD>
+template <int K, bool GRAD, typename W>
 __device__ __forceinline__ void general_weights(const DimParams& dp, double x, int r, int* bad,
-                                                DimW<K, int64_t>& dd) {
+                                                DimW<K, int64_t, W>& dd) {
     wrap_or_check(dp, x, bad);
     const int last = dp.nt - K - 2;
     const int mu0 = dp.uniform ? hint<K>(dp, x, last) : K;
@@ -165,10 +169,10 @@ __device__ __forceinline__ void general_weights(const DimParams& dp, double x, i
     triangle<K>(dp, mu, x, b, keep, GRAD ? K - 1 : K - r, R);
     if constexpr (GRAD) {
 #pragma unroll
-        for (int m = 0; m <= K; ++m) dd.w[m] = b[m];
+        for (int m = 0; m <= K; ++m) dd.w[m] = static_cast<W>(b[m]);
         if (K >= 1) differentiate<K>(keep, 1, R);
 #pragma unroll
-        for (int m = 0; m <= K; ++m) dd.dw[m] = K >= 1 ? keep[m] : 0.0;
+        for (int m = 0; m <= K; ++m) dd.dw[m] = static_cast<W>(K >= 1 ? keep[m] : 0.0);
     } else {
         if (r == 0) {
 #pragma unroll
@@ -177,7 +181,7 @@ __device__ __forceinline__ void general_weights(const DimParams& dp, double x, i
             differentiate<K>(keep, r, R);
         }
 #pragma unroll
-        for (int m = 0; m <= K; ++m) dd.w[m] = keep[m];
+        for (int m = 0; m <= K; ++m) dd.w[m] = static_cast<W>(keep[m]);
     }
     stencil_offsets<K>(dp, mu, dd.off);
 }
@@ -198,22 +202,22 @@ __device__ __forceinline__ int uniform_span(const DimParams& dp, double x) {
 
 // Weights from the span's basis polynomials P (row j: coefficients of
 // basis j in u = (x - t_mu) / dx, ascending powers).
-template <int K, bool GRAD>
+template <int K, bool GRAD, typename W>
 __device__ __forceinline__ void poly_weights(const double* __restrict__ P, double u, double inv_dx, int r,
-                                             DimW<K, uint32_t>& dd) {
-    constexpr int W = K + 1;
+                                             DimW<K, uint32_t, W>& dd) {
+    constexpr int NW = K + 1;
     if constexpr (GRAD) {
 #pragma unroll
         for (int j = 0; j <= K; ++j) {
-            double v = P[j * W + K];
-            double g = static_cast<double>(K) * P[j * W + K];
+            double v = P[j * NW + K];
+            double g = static_cast<double>(K) * P[j * NW + K];
 #pragma unroll
             for (int q = K - 1; q >= 0; --q) {
-                v = v * u + P[j * W + q];
-                if (q >= 1) g = g * u + static_cast<double>(q) * P[j * W + q];
+                v = v * u + P[j * NW + q];
+                if (q >= 1) g = g * u + static_cast<double>(q) * P[j * NW + q];
             }
-            dd.w[j] = v;
-            dd.dw[j] = K >= 1 ? g * inv_dx : 0.0;
+            dd.w[j] = static_cast<W>(v);
+            dd.dw[j] = static_cast<W>(K >= 1 ? g * inv_dx : 0.0);
         }
     } else {
         // r-th derivative: sum_q q!/(q-r)! P_q u^(q-r), times dx^-r
@@ -229,10 +233,10 @@ __device__ __forceinline__ void poly_weights(const double* __restrict__ P, doubl
 #pragma unroll
                     for (int t = 0; t < K; ++t)
                         if (t < r) ff *= static_cast<double>(q - t);
-                    v = v * u + ff * P[j * W + q];
+                    v = v * u + ff * P[j * NW + q];
                 }
             }
-            dd.w[j] = r > K ? 0.0 : v * scale;
+            dd.w[j] = static_cast<W>(r > K ? 0.0 : v * scale);
         }
     }
 }
@@ -242,18 +246,18 @@ __device__ __forceinline__ void poly_weights(const double* __restrict__ P, doubl
 // lane of the wave is on an interior span (class 0, the common case) the
 // coefficients are read with scalar loads from dp.poly instead (the same
 // values), which keeps them out of the LDS pipe.
-template <int K, bool GRAD>
+template <int K, bool GRAD, typename W>
 __device__ __forceinline__ int uniform_weights(const DimParams& dp, const double* __restrict__ poly, double x,
-                                               int r, int* bad, DimW<K, uint32_t>& dd) {
+                                               int r, int* bad, DimW<K, uint32_t, W>& dd) {
     wrap_or_check(dp, x, bad);
     const int last = dp.nt - K - 2;
     const int mu = uniform_span<K>(dp, x);
     const int cls = (mu - K < dp.nleft) ? 1 + (mu - K) : ((last - mu < dp.nright) ? 1 + dp.nleft + (last - mu) : 0);
     const double u = (x - uknot_sel<K>(dp, mu)) * dp.inv_dx;
     if (__all(cls == 0))
-        poly_weights<K, GRAD>(dp.poly, u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W>(dp.poly, u, dp.inv_dx, r, dd);
     else
-        poly_weights<K, GRAD>(poly + cls * (K + 1) * (K + 1), u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W>(poly + cls * (K + 1) * (K + 1), u, dp.inv_dx, r, dd);
     stencil_offsets<K>(dp, mu, dd.off);
     return mu;
 }
@@ -265,15 +269,15 @@ __device__ __forceinline__ int uniform_weights(const DimParams& dp, const double
 // One stencil row along the last dimension.  LASTC: the last dimension is
 // not periodic, so the row is K+1 consecutive elements from one address
 // (one address computation, immediate offsets).
-template <int K, bool LASTC, typename Off>
-__device__ __forceinline__ void load_row(const double* __restrict__ c, Off rowoff, const DimW<K, Off>& dl,
-                                         double (&cv)[K + 1]) {
+template <int K, bool LASTC, typename T, typename Off>
+__device__ __forceinline__ void load_row(const T* __restrict__ c, Off rowoff, const DimW<K, Off, T>& dl,
+                                         T (&cv)[K + 1]) {
     if constexpr (LASTC) {
-        const double* __restrict__ r = c + (rowoff + dl.off[0]);
+        const T* __restrict__ r = c + (rowoff + dl.off[0]);
 #pragma unroll
         for (int e = 0; e <= K; ++e) cv[e] = r[e];
     } else {
-        const double* __restrict__ r = c + rowoff;
+        const T* __restrict__ r = c + rowoff;
 #pragma unroll
         for (int e = 0; e <= K; ++e) cv[e] = r[dl.off[e]];
     }
@@ -282,8 +286,8 @@ __device__ __forceinline__ void load_row(const double* __restrict__ c, Off rowof
 // Shifts the weights/offsets of the first dimension down by one, so a
 // rolled loop over its K+1 planes only ever reads element 0 (registers,
 // never scratch).
-template <int K, typename Off>
-__device__ __forceinline__ void shift_down(DimW<K, Off>& d) {
+template <int K, typename Off, typename W>
+__device__ __forceinline__ void shift_down(DimW<K, Off, W>& d) {
 #pragma unroll
     for (int m = 0; m < K; ++m) {
         d.w[m] = d.w[m + 1];
@@ -292,13 +296,12 @@ __device__ __forceinline__ void shift_down(DimW<K, Off>& d) {
     }
 }
 
-template <int D, int K, bool LASTC, typename Off>
-__device__ __forceinline__ void contract_grad(const double* __restrict__ c, DimW<K, Off> (&dd)[D],
-                                              double (&res)[D + 1]) {
+template <int D, int K, bool LASTC, typename T, typename Off>
+__device__ __forceinline__ void contract_grad(const T* __restrict__ c, DimW<K, Off, T> (&dd)[D], T (&res)[D + 1]) {
     if constexpr (D == 1) {
-        double cv[K + 1];
-        load_row<K, LASTC>(c, Off(0), dd[0], cv);
-        double v = 0.0, g = 0.0;
+        T cv[K + 1];
+        load_row<K, LASTC, T>(c, Off(0), dd[0], cv);
+        T v = T(0), g = T(0);
 #pragma unroll
         for (int a = 0; a <= K; ++a) {
             v += dd[0].w[a] * cv[a];
@@ -307,12 +310,12 @@ __device__ __forceinline__ void contract_grad(const double* __restrict__ c, DimW
         res[0] = v;
         res[1] = g;
     } else if constexpr (D == 2) {
-        double v = 0.0, g0 = 0.0, g1 = 0.0;
+        T v = T(0), g0 = T(0), g1 = T(0);
 #pragma unroll
         for (int a = 0; a <= K; ++a) {
-            double cv[K + 1];
-            load_row<K, LASTC>(c, dd[0].off[a], dd[1], cv);
-            double sB = 0.0, sD = 0.0;
+            T cv[K + 1];
+            load_row<K, LASTC, T>(c, dd[0].off[a], dd[1], cv);
+            T sB = T(0), sD = T(0);
 #pragma unroll
             for (int b = 0; b <= K; ++b) {
                 sB += dd[1].w[b] * cv[b];
@@ -326,16 +329,16 @@ __device__ __forceinline__ void contract_grad(const double* __restrict__ c, DimW
         res[1] = g0;
         res[2] = g1;
     } else {
-        double v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        T v = T(0), g0 = T(0), g1 = T(0), g2 = T(0);
         constexpr int kOuterUnroll = K <= INTP_OUTER_UNROLL_MAX_K ? K + 1 : 1;
 #pragma unroll kOuterUnroll
         for (int a = 0; a <= K; ++a) {
-            double tBB = 0.0, tDB = 0.0, tBD = 0.0;
+            T tBB = T(0), tDB = T(0), tBD = T(0);
 #pragma unroll
             for (int b = 0; b <= K; ++b) {
-                double cv[K + 1];
-                load_row<K, LASTC>(c, dd[0].off[0] + dd[1].off[b], dd[2], cv);
-                double sB = 0.0, sD = 0.0;
+                T cv[K + 1];
+                load_row<K, LASTC, T>(c, dd[0].off[0] + dd[1].off[b], dd[2], cv);
+                T sB = T(0), sD = T(0);
 #pragma unroll
                 for (int e = 0; e <= K; ++e) {
                     sB += dd[2].w[e] * cv[e];
@@ -349,7 +352,7 @@ __device__ __forceinline__ void contract_grad(const double* __restrict__ c, DimW
             g0 += dd[0].dw[0] * tBB;
             g1 += dd[0].w[0] * tDB;
             g2 += dd[0].w[0] * tBD;
-            shift_down<K>(dd[0]);
+            shift_down<K, Off, T>(dd[0]);
         }
         res[0] = v;
         res[1] = g0;
@@ -358,20 +361,20 @@ __device__ __forceinline__ void contract_grad(const double* __restrict__ c, DimW
     }
 }
 
-template <int D, int K, bool LASTC, typename Off>
-__device__ __forceinline__ double contract_single(const double* __restrict__ c, DimW<K, Off> (&dd)[D]) {
-    double v = 0.0;
+template <int D, int K, bool LASTC, typename T, typename Off>
+__device__ __forceinline__ T contract_single(const T* __restrict__ c, DimW<K, Off, T> (&dd)[D]) {
+    T v = T(0);
     if constexpr (D == 1) {
-        double cv[K + 1];
-        load_row<K, LASTC>(c, Off(0), dd[0], cv);
+        T cv[K + 1];
+        load_row<K, LASTC, T>(c, Off(0), dd[0], cv);
 #pragma unroll
         for (int a = 0; a <= K; ++a) v += dd[0].w[a] * cv[a];
     } else if constexpr (D == 2) {
 #pragma unroll
         for (int a = 0; a <= K; ++a) {
-            double cv[K + 1];
-            load_row<K, LASTC>(c, dd[0].off[a], dd[1], cv);
-            double sum = 0.0;
+            T cv[K + 1];
+            load_row<K, LASTC, T>(c, dd[0].off[a], dd[1], cv);
+            T sum = T(0);
 #pragma unroll
             for (int b = 0; b <= K; ++b) sum += dd[1].w[b] * cv[b];
             v += dd[0].w[a] * sum;
@@ -380,27 +383,35 @@ __device__ __forceinline__ double contract_single(const double* __restrict__ c, 
         constexpr int kOuterUnroll = K <= INTP_OUTER_UNROLL_MAX_K ? K + 1 : 1;
 #pragma unroll kOuterUnroll
         for (int a = 0; a <= K; ++a) {
-            double tsum = 0.0;
+            T tsum = T(0);
 #pragma unroll
             for (int b = 0; b <= K; ++b) {
-                double cv[K + 1];
-                load_row<K, LASTC>(c, dd[0].off[0] + dd[1].off[b], dd[2], cv);
-                double sum = 0.0;
+                T cv[K + 1];
+                load_row<K, LASTC, T>(c, dd[0].off[0] + dd[1].off[b], dd[2], cv);
+                T sum = T(0);
 #pragma unroll
                 for (int e = 0; e <= K; ++e) sum += dd[2].w[e] * cv[e];
                 tsum += dd[1].w[b] * sum;
             }
             v += dd[0].w[0] * tsum;
-            shift_down<K>(dd[0]);
+            shift_down<K, Off, T>(dd[0]);
         }
     }
     return v;
 }
 
 // Stores the value and first partials of one point.
-template <int D>
-__device__ __forceinline__ void store_grad(const EvalParams& p, int64_t opt, const double (&res)[D + 1]) {
-    if (p.out_packed) {
+// One output value (FP64, or FP32 into out32).
+__device__ __forceinline__ void store_out(const EvalParams& p, int64_t at, double v) {
+    if (p.out32)
+        p.out32[at] = static_cast<float>(v);
+    else
+        p.out[at] = v;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void store_grad(const EvalParams& p, int64_t opt, const T (&res)[D + 1]) {
+    if (p.out_packed && !p.out32) {
         double* o = p.out + opt * p.os_pt;
 #pragma unroll
         for (int s = 0; s + 1 <= D; s += 2) *reinterpret_cast<double2*>(o + s) = make_double2(res[s], res[s + 1]);
@@ -408,7 +419,7 @@ __device__ __forceinline__ void store_grad(const EvalParams& p, int64_t opt, con
     } else {
 #pragma unroll
         for (int s = 0; s <= D; ++s)
-            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
+            if (p.slot[s] >= 0) store_out(p, p.slot[s] * p.os_out + opt * p.os_pt, res[s]);
     }
 }
 
@@ -421,58 +432,60 @@ __device__ __forceinline__ void store_grad(const EvalParams& p, int64_t opt, con
 // chunks: with grouped points, one L2 then holds each brick patch.
 __device__ __forceinline__ int64_t block_index() { return xcd_chunk_index(); }
 
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void general_point(const EvalParams& p, int64_t pt) {
     const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
-    DimW<K, int64_t> dd[D];
+    const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
+    DimW<K, int64_t, T> dd[D];
     bool zero = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        general_weights<K, GRAD>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+        general_weights<K, GRAD, T>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
     }
     if constexpr (GRAD) {
-        double res[D + 1];
-        contract_grad<D, K, LASTC>(p.ctrl, dd, res);
+        T res[D + 1];
+        contract_grad<D, K, LASTC>(ctrl, dd, res);
 #pragma unroll
         for (int s = 0; s <= D; ++s)
-            if (p.slot[s] >= 0) p.out[p.slot[s] * p.os_out + opt * p.os_pt] = res[s];
+            if (p.slot[s] >= 0) store_out(p, p.slot[s] * p.os_out + opt * p.os_pt, res[s]);
     } else {
-        const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
-        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : v;
+        const T v = contract_single<D, K, LASTC>(ctrl, dd);
+        store_out(p, p.out_index * p.os_out + opt * p.os_pt, zero ? 0.0 : static_cast<double>(v));
     }
 }
 
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __global__ __launch_bounds__(kBlock) void eval_general_kernel(EvalParams p) {
     const int64_t pt = block_index() * kBlock + threadIdx.x;
-    if (pt < p.npts) general_point<D, K, GRAD, LASTC>(p, pt);
+    if (pt < p.npts) general_point<D, K, GRAD, LASTC, T>(p, pt);
 }
 
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void uniform_point(const EvalParams& p, const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
                                               int64_t pt) {
     const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
-    DimW<K, uint32_t> dd[D];
+    const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
+    DimW<K, uint32_t, T> dd[D];
     bool zero = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        uniform_weights<K, GRAD>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+        uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
     }
     if constexpr (GRAD) {
-        double res[D + 1];
-        contract_grad<D, K, LASTC>(p.ctrl, dd, res);
+        T res[D + 1];
+        contract_grad<D, K, LASTC>(ctrl, dd, res);
         store_grad<D>(p, opt, res);
     } else {
-        const double v = contract_single<D, K, LASTC>(p.ctrl, dd);
-        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : v;
+        const T v = contract_single<D, K, LASTC>(ctrl, dd);
+        store_out(p, p.out_index * p.os_out + opt * p.os_pt, zero ? 0.0 : static_cast<double>(v));
     }
 }
 
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
     constexpr int kClassMax = 1 + 2 * K;
     constexpr int kCoef = (K + 1) * (K + 1);
@@ -484,31 +497,36 @@ __global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
     }
     __syncthreads();
     const int64_t pt = block_index() * kBlock + threadIdx.x;
-    if (pt < p.npts) uniform_point<D, K, GRAD, LASTC>(p, s_poly, pt);
+    if (pt < p.npts) uniform_point<D, K, GRAD, LASTC, T>(p, s_poly, pt);
 }
 
 // ---- grouped evaluation with the brick patch in LDS ---------------------
 
-constexpr int kPatchMax = 2048;  // doubles of one brick patch (16 KB)
+constexpr int kPatchBytes = 16384;  // one brick patch in LDS
+template <typename T>
+constexpr int patch_max() {
+    return kPatchBytes / static_cast<int>(sizeof(T));
+}
 
 // One point of a grouped chunk: if its stencil lies in the staged patch
 // (its brick is the chunk's), the stencil is read from LDS, otherwise from
 // HBM.  Same weights and contraction order either way.
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void patch_point(const EvalParams& p,
                                             const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
-                                            const double* s_patch, const int (&bstart)[D], const int (&bsize)[D],
+                                            const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
                                             const uint32_t (&lstride)[D], int64_t pt) {
     const int64_t opt = static_cast<int64_t>(p.perm[pt * p.perm_stride]);
-    DimW<K, uint32_t> dd[D];
+    const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
+    DimW<K, uint32_t, T> dd[D];
     bool zero = false, inside = true;
     int local[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        const int mu = uniform_weights<K, GRAD>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r,
-                                                p.bad, dd[d]);
+        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r,
+                                                   p.bad, dd[d]);
         const int n = static_cast<int>(p.dim[d].n);
         int st = mu - K;  // first stencil point; periodic even orders: may be n (wraps to 0)
         if (st >= n) st -= n;
@@ -517,7 +535,7 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
         local[d] = l;
         inside = inside && l >= 0 && l < bsize[d];
     }
-    double res[D + 1];
+    T res[D + 1];
     if (inside) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
@@ -529,26 +547,28 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
             res[0] = contract_single<D, K, true>(s_patch, dd);
     } else {
         if constexpr (GRAD)
-            contract_grad<D, K, LASTC>(p.ctrl, dd, res);
+            contract_grad<D, K, LASTC>(ctrl, dd, res);
         else
-            res[0] = contract_single<D, K, LASTC>(p.ctrl, dd);
+            res[0] = contract_single<D, K, LASTC>(ctrl, dd);
     }
     if constexpr (GRAD) {
         store_grad<D>(p, opt, res);
     } else {
-        p.out[p.out_index * p.os_out + opt * p.os_pt] = zero ? 0.0 : res[0];
+        store_out(p, p.out_index * p.os_out + opt * p.os_pt, zero ? 0.0 : static_cast<double>(res[0]));
     }
 }
 
 // One workgroup per 256 grouped points: stages the brick of its middle
 // point, (B + K)^D control points (wrapped on periodic dimensions, zero
 // past the end of clamped ones, where no stencil reaches), then evaluates.
-template <int D, int K, bool GRAD, bool LASTC>
+template <int D, int K, bool GRAD, bool LASTC, typename T>
 __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
     constexpr int kClassMax = 1 + 2 * K;
     constexpr int kCoef = (K + 1) * (K + 1);
+    constexpr int kPatchMax = patch_max<T>();
+    const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
     __shared__ double s_poly[D][kClassMax * kCoef];
-    __shared__ double s_patch[kPatchMax];
+    __shared__ T s_patch[kPatchMax];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int n = p.dim[d].nclass * kCoef;
@@ -574,7 +594,7 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
         lstride[d] = vol;
         vol *= static_cast<uint32_t>(psize[d]);
     }
-    if (vol > static_cast<uint32_t>(kPatchMax) || psize[D - 1] > (D == 3 ? 16 : 64) || (D == 3 && psize[1] > 16) ||
+    if (vol > static_cast<uint32_t>(kPatchMax) || psize[D - 1] > (D == 3 ? 16 : 64) || (D == 3 && psize[D == 3 ? 1 : 0] > 16) ||
         psize[0] > (D == 3 ? 16 : 64)) {
         // not staged: every point reads HBM
         vol = 0;
@@ -621,15 +641,15 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
             const int n0 = static_cast<int>(p.dim[0].n);
             const bool per0 = p.dim[0].periodic != 0;
             const uint32_t stride0 = static_cast<uint32_t>(p.dim[0].stride);
-            double v[kRows];
+            T v[kRows];
 #pragma unroll
             for (int r = 0; r < kRows; ++r) {
                 const int i0 = group + r * kGroups;
-                v[r] = 0.0;
+                v[r] = T(0);
                 if (i0 < psize[0]) {
                     int g0 = bstart[0] + i0;
                     if (g0 >= n0) g0 = per0 ? g0 % n0 : -1;
-                    if (g0 >= 0) v[r] = p.ctrl[inner_off + static_cast<uint32_t>(g0) * stride0];
+                    if (g0 >= 0) v[r] = ctrl[inner_off + static_cast<uint32_t>(g0) * stride0];
                 }
             }
 #pragma unroll
@@ -643,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
     }
     __syncthreads();
     const int64_t pt = first + threadIdx.x;
-    if (pt < p.npts) patch_point<D, K, GRAD, LASTC>(p, s_poly, s_patch, bstart, bsize, lstride, pt);
+    if (pt < p.npts) patch_point<D, K, GRAD, LASTC, T>(p, s_poly, s_patch, bstart, bsize, lstride, pt);
 }
 
 template <void (*KF)(EvalParams)>
@@ -655,60 +675,45 @@ hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int D, int K, bool LASTC>
+template <int D, int K, bool LASTC, typename T>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
     if (uniform && p.patch_volume > 0) {
-        if (D < 2 || p.patch_volume > kPatchMax || !p.perm || !p.key) return hipErrorInvalidValue;
-        if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC>>(p, s);
-        return launch_chunks<eval_patch_kernel<D, K, false, LASTC>>(p, s);
+        if (D < 2 || p.patch_volume > patch_max<T>() || !p.perm || !p.key) return hipErrorInvalidValue;
+        if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC, T>>(p, s);
+        return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
     }
     if (uniform) {
-        if (grad) return launch_chunks<eval_uniform_kernel<D, K, true, LASTC>>(p, s);
-        return launch_chunks<eval_uniform_kernel<D, K, false, LASTC>>(p, s);
+        if (grad) return launch_chunks<eval_uniform_kernel<D, K, true, LASTC, T>>(p, s);
+        return launch_chunks<eval_uniform_kernel<D, K, false, LASTC, T>>(p, s);
     }
-    if (grad) return launch_chunks<eval_general_kernel<D, K, true, LASTC>>(p, s);
-    return launch_chunks<eval_general_kernel<D, K, false, LASTC>>(p, s);
+    if (grad) return launch_chunks<eval_general_kernel<D, K, true, LASTC, T>>(p, s);
+    return launch_chunks<eval_general_kernel<D, K, false, LASTC, T>>(p, s);
 }
 
-template <int D, int K>
+template <int D, int K, typename T>
 hipError_t launch_one(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
     // contiguous stencil rows need a non-periodic last dimension (stride 1)
-    if (p.dim[D - 1].periodic) return launch_variant<D, K, false>(grad, uniform, p, s);
-    return launch_variant<D, K, true>(grad, uniform, p, s);
-}
-
-template <int D>
-hipError_t dispatch_order(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
-    switch (order) {
-        case 0: return launch_one<D, 0>(grad, uniform, p, s);
-        case 1: return launch_one<D, 1>(grad, uniform, p, s);
-        case 2: return launch_one<D, 2>(grad, uniform, p, s);
-        case 3: return launch_one<D, 3>(grad, uniform, p, s);
-        case 4: return launch_one<D, 4>(grad, uniform, p, s);
-        case 5: return launch_one<D, 5>(grad, uniform, p, s);
-        case 6: return launch_one<D, 6>(grad, uniform, p, s);
-        case 7: return launch_one<D, 7>(grad, uniform, p, s);
-        default: return hipErrorNotSupported;
-    }
-}
-
-hipError_t dispatch(int dim, int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
-    switch (dim) {
-        case 1: return dispatch_order<1>(order, grad, uniform, p, s);
-        case 2: return dispatch_order<2>(order, grad, uniform, p, s);
-        case 3: return dispatch_order<3>(order, grad, uniform, p, s);
-        default: return hipErrorNotSupported;
-    }
+    if (p.dim[D - 1].periodic) return launch_variant<D, K, false, T>(grad, uniform, p, s);
+    return launch_variant<D, K, true, T>(grad, uniform, p, s);
 }
 
 }  // namespace
 
-hipError_t launch_eval_grad(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s) {
-    return dispatch(dim, order, true, uniform, p, s);
-}
-
-hipError_t launch_eval_single(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s) {
-    return dispatch(dim, order, false, uniform, p, s);
+// Kernels of dimension D with table type T, all orders (one translation
+// unit per (T, D) instantiates this).
+template <int D, typename T>
+hipError_t eval_dispatch(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    switch (order) {
+        case 0: return launch_one<D, 0, T>(grad, uniform, p, s);
+        case 1: return launch_one<D, 1, T>(grad, uniform, p, s);
+        case 2: return launch_one<D, 2, T>(grad, uniform, p, s);
+        case 3: return launch_one<D, 3, T>(grad, uniform, p, s);
+        case 4: return launch_one<D, 4, T>(grad, uniform, p, s);
+        case 5: return launch_one<D, 5, T>(grad, uniform, p, s);
+        case 6: return launch_one<D, 6, T>(grad, uniform, p, s);
+        case 7: return launch_one<D, 7, T>(grad, uniform, p, s);
+        default: return hipErrorNotSupported;
+    }
 }
 
 }  // namespace intp_amd

@@ -43,10 +43,11 @@ struct BrickGrid {
 
 struct EvalParams {
     DimParams dim[kMaxDim];
-    const double* ctrl;
+    const void* ctrl;  // control points: double, or float for a refined build
     const double* coords;
     int64_t cs_dim, cs_pt;
     double* out;
+    float* out32;  // set instead of `out`: results stored as FP32
     int64_t os_out, os_pt;
     int64_t npts;
     // gradient kernels: output index written for value (slot 0) and for
@@ -97,18 +98,27 @@ struct BinParams {
 
 // Line solves along one axis of a C-order table viewed as
 // (outer, n, inner): line (o, i) holds elements o*n*inner + j*inner + i.
+// Element types of a line solve: storage (the table) and compute (the
+// factors, the LDS tile and the substitution).
+enum class SolveTypes { kF64, kF32, kF32StoreF64Compute };
+
 struct SolveParams {
-    double* x;  // solved in place
+    void* x;  // the table (storage type), solved in place
     // First pass only (last axis, inner == 1): read the right-hand sides
-    // from `src` with the periodic shift applied, element (i_0..i_{D-1}) of
-    // x taking source element ((i_d - shift_d) mod n_d); nullptr: in place.
+    // from `src` (FP64) with the periodic shift applied, element
+    // (i_0..i_{D-1}) of x taking source element ((i_d - shift_d) mod n_d);
+    // nullptr: in place.
     const double* src;
+    // If set (LDS-tiled kernel only): the solved lines are added into
+    // accum (storage type) instead of being stored to x.
+    void* accum;
     int64_t n, inner, nlines;
-    const double* lower;   // n * bw
-    const double* upper;   // n * bw
-    const double* diag;    // n
-    const double* bottom;  // bw * n (cyclic)
-    const double* right;   // bw * n (cyclic)
+    // factors in the compute type
+    const void* lower;   // n * bw
+    const void* upper;   // n * bw
+    const void* diag;    // n
+    const void* bottom;  // bw * n (cyclic)
+    const void* right;   // bw * n (cyclic)
     int32_t outer_dims;            // dims before the solved (last) axis
     int64_t outer_n[kMaxDim];      // their sizes
     int64_t outer_shift[kMaxDim];  // their shifts
@@ -118,7 +128,7 @@ struct SolveParams {
 
 struct ShiftParams {
     const double* src;
-    double* dst;
+    void* dst;  // double, or float (launch_shift_copy dst_f32)
     int64_t total;
     int32_t dim;
     int64_t n[kMaxDim];
@@ -142,12 +152,21 @@ __device__ __forceinline__ int64_t xcd_chunk_index() {
 // Returns hipSuccess or the launch error; hipErrorNotSupported when no
 // kernel is compiled for (dim, order).  `uniform`: every dimension is
 // uniform and the table has fewer than 2^32 elements (fast path).
-hipError_t launch_eval_grad(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
-hipError_t launch_eval_single(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s);
-hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s);
+hipError_t launch_eval(int dim, int order, bool grad, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
+hipError_t launch_eval_grad(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
+hipError_t launch_eval_single(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
+// per table type and dimension (eval_f64_d*.hip, eval_f32_d*.hip)
+hipError_t eval_dispatch_f64_d1(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f64_d2(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f64_d3(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d1(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d2(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d3(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t launch_solve(int bw, bool cyclic, SolveTypes types, const SolveParams& p, hipStream_t s);
 // Lines of length n that one LDS-tiled solve workgroup holds on the
-// current device (a power of two up to 64), 0 if a line does not fit.
-int solve_tile_lines(int64_t n);
+// current device (a power of two up to 64) for compute elements of
+// `compute_bytes`, 0 if a line does not fit.
+int solve_tile_lines(int64_t n, int compute_bytes);
 
 // Grouping of query points by brick (bin.hip): scratch sizes and launch.
 int64_t bin_chunk_points();
@@ -155,6 +174,23 @@ int64_t bin_hist_entries(int64_t npts);
 int64_t bin_scan_tiles(int64_t npts);
 hipError_t launch_binning(BinParams p, uint32_t* hist, uint32_t* offs, uint32_t* tile_sums, uint32_t* keys,
                           BinRecord* rec_a, BinRecord* rec_b, hipStream_t s);
-hipError_t launch_shift_copy(const ShiftParams& p, hipStream_t s);
+hipError_t launch_shift_copy(const ShiftParams& p, bool dst_f32, hipStream_t s);
+
+// y = M x along one axis of a table viewed as (outer, n, inner), M the
+// axis' collocation matrix as rows colloc[j*(2w+1) + (m+w)] = M(j, j+m)
+// (columns mod n if periodic); with src set, y = (shifted src) - M x.
+struct MatvecParams {
+    const void* x;  // float or double
+    void* y;        // float or double
+    const double* src;
+    const double* colloc;
+    int32_t w, periodic;
+    int64_t n, inner, total;
+    int32_t dim;
+    int64_t dn[kMaxDim];
+    int64_t dshift[kMaxDim];
+};
+hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s);
+hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s);
 
 }  // namespace intp_amd

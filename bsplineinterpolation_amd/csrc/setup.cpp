@@ -309,8 +309,27 @@ std::string setup_dimension(int order, bool periodic, int64_t n, double xmin, do
             m.at(row, col) = bv[j];
         }
     }
+    // the unfactored matrix, row-wise, for FP64 residuals
+    s.colloc_w = bw;
+    s.colloc.assign(static_cast<size_t>(n * (2 * bw + 1)), 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int mm = -bw; mm <= bw; ++mm) {
+            int64_t j = i + mm;
+            if (periodic) j = ((j % n) + n) % n;
+            if (j < 0 || j >= n || !m.stored(i, j)) continue;
+            s.colloc[static_cast<size_t>(i * (2 * bw + 1) + (mm + bw))] = m.at(i, j);
+        }
     m.factor();
     m.export_factors(s.lu);
+    auto round32 = [](const std::vector<double>& v, std::vector<float>& out) {
+        out.resize(v.size());
+        for (size_t i = 0; i < v.size(); ++i) out[i] = static_cast<float>(v[i]);
+    };
+    round32(s.lu.lower, s.lower32);
+    round32(s.lu.upper, s.upper32);
+    round32(s.lu.diag, s.diag32);
+    round32(s.lu.bottom, s.bottom32);
+    round32(s.lu.right, s.right32);
     for (int64_t i = 0; i < n; ++i)
         if (!(std::fabs(s.lu.diag[static_cast<size_t>(i)]) > 0.0) || !std::isfinite(s.lu.diag[static_cast<size_t>(i)]))
             return "singular collocation matrix";

@@ -33,6 +33,14 @@ struct DimSetup {
     std::vector<double> nodes;    // non-uniform node coordinates
     double lo = 0, hi = 0, dx = 0;  // range and uniform spacing
     LineFactors lu;
+    // The collocation matrix itself (before LU), for residuals of the
+    // refined-FP32 build: colloc[i*(2w+1) + (m+w)] = A(i, i+m), columns
+    // taken mod n on periodic dimensions (zero outside a clamped matrix),
+    // m = -w..w, w = colloc_w.
+    int colloc_w = 0;
+    std::vector<double> colloc;
+    // The LU factors rounded to FP32 (same layout as `lu`).
+    std::vector<float> lower32, upper32, diag32, bottom32, right32;
     // reciprocal knot differences for the basis recursion, per span mu:
     // recip[mu*T + i*(i-1)/2 + a-1] = 1 / (t[mu+a] - t[mu+a-i]),
     // i = 1..order, a = 1..i, T = order*(order+1)/2

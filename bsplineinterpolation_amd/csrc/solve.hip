@@ -37,37 +37,39 @@ constexpr int kMaxTileLines = 64;
 // stored.
 constexpr int kBatch = 8;
 
-template <int BW, bool CYC>
+template <int BW, bool CYC, typename C>
 struct LineState {
-    double hist[BW + 1];    // forward: hist[m-1] = y[j-m]
-    double acc[BW + 1];     // forward: rows n-BW+r of a cyclic matrix, strip-updated
-    double nxt[BW + 1];     // backward: nxt[m-1] = x[j+m]
-    double corner[BW + 1];  // backward: x[n-BW+c]
+    C hist[BW + 1];    // forward: hist[m-1] = y[j-m]
+    C acc[BW + 1];     // forward: rows n-BW+r of a cyclic matrix, strip-updated
+    C nxt[BW + 1];     // backward: nxt[m-1] = x[j+m]
+    C corner[BW + 1];  // backward: x[n-BW+c]
 };
 
 // Forward substitution of rows j0 .. j0+B-1 (unit lower triangle, plus
 // the bottom strip of a cyclic matrix).
-template <int BW, bool CYC, int B>
-__device__ __forceinline__ void forward_rows(double* __restrict__ x, int64_t step, int64_t n, int64_t j0,
-                                             const SolveParams& sp, LineState<BW, CYC>& st) {
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void forward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
+                                             const SolveParams& sp, LineState<BW, CYC, C>& st) {
 #pragma clang fp contract(off)
-    double xb[B];
-    double lc[B][BW + 1];
-    double sc[B][BW + 1];
+    const C* __restrict__ lower = static_cast<const C*>(sp.lower);
+    const C* __restrict__ bottom = static_cast<const C*>(sp.bottom);
+    C xb[B];
+    C lc[B][BW + 1];
+    C sc[B][BW + 1];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-        xb[u] = x[(j0 + u) * step];
+        xb[u] = static_cast<C>(x[(j0 + u) * step]);
 #pragma unroll
-        for (int m = 0; m < BW; ++m) lc[u][m] = sp.lower[(j0 + u) * BW + m];
+        for (int m = 0; m < BW; ++m) lc[u][m] = lower[(j0 + u) * BW + m];
         if constexpr (CYC) {
 #pragma unroll
-            for (int r = 0; r < BW; ++r) sc[u][r] = sp.bottom[r * n + j0 + u];
+            for (int r = 0; r < BW; ++r) sc[u][r] = bottom[r * n + j0 + u];
         }
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 + u;
-        double v = xb[u];
+        C v = xb[u];
         if constexpr (CYC) {
 #pragma unroll
             for (int r = 0; r < BW; ++r)
@@ -86,35 +88,38 @@ __device__ __forceinline__ void forward_rows(double* __restrict__ x, int64_t ste
         }
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u) x[(j0 + u) * step] = xb[u];
+    for (int u = 0; u < B; ++u) x[(j0 + u) * step] = static_cast<P>(xb[u]);
 }
 
 // Backward substitution of rows j0, j0-1, .., j0-B+1 (upper triangle,
 // plus the right strip of a cyclic matrix).
-template <int BW, bool CYC, int B>
-__device__ __forceinline__ void backward_rows(double* __restrict__ x, int64_t step, int64_t n, int64_t j0,
-                                              const SolveParams& sp, LineState<BW, CYC>& st) {
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void backward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
+                                              const SolveParams& sp, LineState<BW, CYC, C>& st) {
 #pragma clang fp contract(off)
-    double xb[B];
-    double uc[B][BW + 1];
-    double rc[B][BW + 1];
-    double dg[B];
+    const C* __restrict__ upper = static_cast<const C*>(sp.upper);
+    const C* __restrict__ diag = static_cast<const C*>(sp.diag);
+    const C* __restrict__ right = static_cast<const C*>(sp.right);
+    C xb[B];
+    C uc[B][BW + 1];
+    C rc[B][BW + 1];
+    C dg[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 - u;
-        xb[u] = x[j * step];
+        xb[u] = static_cast<C>(x[j * step]);
 #pragma unroll
-        for (int m = 0; m < BW; ++m) uc[u][m] = sp.upper[j * BW + m];
-        dg[u] = sp.diag[j];
+        for (int m = 0; m < BW; ++m) uc[u][m] = upper[j * BW + m];
+        dg[u] = diag[j];
         if constexpr (CYC) {
 #pragma unroll
-            for (int c = 0; c < BW; ++c) rc[u][c] = sp.right[c * n + j];
+            for (int c = 0; c < BW; ++c) rc[u][c] = right[c * n + j];
         }
     }
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 - u;
-        double v = xb[u];
+        C v = xb[u];
         if constexpr (CYC) {
 #pragma unroll
             for (int c = BW - 1; c >= 0; --c)
@@ -134,44 +139,51 @@ __device__ __forceinline__ void backward_rows(double* __restrict__ x, int64_t st
         }
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u) x[(j0 - u) * step] = xb[u];
+    for (int u = 0; u < B; ++u) x[(j0 - u) * step] = static_cast<P>(xb[u]);
 }
 
-// Solves one line x[0], x[step], ..., x[(n-1)*step] in place.
-template <int BW, bool CYC>
-__device__ __forceinline__ void substitute(double* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
-    LineState<BW, CYC> st;
+// Solves one line x[0], x[step], ..., x[(n-1)*step] in place, computing
+// in C (elements of x have type P).
+template <int BW, bool CYC, typename C, typename P>
+__device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
+    LineState<BW, CYC, C> st;
 #pragma unroll
-    for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = 0.0;
+    for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = C(0);
     if constexpr (CYC) {
 #pragma unroll
-        for (int r = 0; r < BW; ++r) st.acc[r] = x[(n - BW + r) * step];
+        for (int r = 0; r < BW; ++r) st.acc[r] = static_cast<C>(x[(n - BW + r) * step]);
     }
     int64_t j = 0;
-    for (; j + kBatch <= n; j += kBatch) forward_rows<BW, CYC, kBatch>(x, step, n, j, sp, st);
-    for (; j < n; ++j) forward_rows<BW, CYC, 1>(x, step, n, j, sp, st);
+    for (; j + kBatch <= n; j += kBatch) forward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
+    for (; j < n; ++j) forward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
     j = n - 1;
-    for (; j - kBatch + 1 >= 0; j -= kBatch) backward_rows<BW, CYC, kBatch>(x, step, n, j, sp, st);
-    for (; j >= 0; --j) backward_rows<BW, CYC, 1>(x, step, n, j, sp, st);
+    for (; j - kBatch + 1 >= 0; j -= kBatch) backward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
+    for (; j >= 0; --j) backward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
 }
 
-// Fallback: one lane per line, substituting in HBM.
-template <int BW, bool CYC>
+// Fallback: one lane per line, substituting in HBM (storage T, compute C;
+// with T narrower than C the forward results are rounded to T between
+// the two sweeps).
+template <int BW, bool CYC, typename T, typename C>
 __global__ __launch_bounds__(kBlock) void solve_kernel(SolveParams sp) {
     const int64_t line = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (line >= sp.nlines) return;
     const int64_t o = line / sp.inner, ii = line - o * sp.inner;
-    substitute<BW, CYC>(sp.x + o * sp.n * sp.inner + ii, sp.inner, sp.n, sp);
+    substitute<BW, CYC, C, T>(static_cast<T*>(sp.x) + o * sp.n * sp.inner + ii, sp.inner, sp.n, sp);
 }
 
-// Odd LDS row stride (in doubles), so that lanes reading the same column
+// Odd LDS row stride (in elements), so that lanes reading the same column
 // of their rows hit distinct banks.
 __host__ __device__ __forceinline__ int64_t tile_stride(int64_t n) { return n | 1; }
 
-// One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS.
-template <int BW, bool CYC>
+// One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS in
+// the compute type C; the table holds T.  With sp.accum set, the solved
+// lines are added into accum instead of being stored to x.
+template <int BW, bool CYC, typename T, typename C>
 __global__ __launch_bounds__(kTileThreads) void solve_tile_kernel(SolveParams sp) {
-    extern __shared__ double tile[];
+    extern __shared__ __align__(16) unsigned char tile_bytes[];
+    C* tile = reinterpret_cast<C*>(tile_bytes);
+    T* __restrict__ X = static_cast<T*>(sp.x);
     __shared__ int64_t dst_base[kMaxTileLines];
     __shared__ int64_t src_base[kMaxTileLines];
     const int lpb = sp.lines_per_block;
@@ -221,19 +233,19 @@ __global__ __launch_bounds__(kTileThreads) void solve_tile_kernel(SolveParams sp
                 from[u] = src_base[l] + js;
                 at[u] = l * static_cast<uint32_t>(stride) + jj;
             }
-            double v[kBatch];
+            C v[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) v[u] = sp.src[from[u]];
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(sp.src[from[u]]);
 #pragma unroll
             for (int u = 0; u < kBatch; ++u)
                 if (e0 + u * kTileThreads < count) tile[at[u]] = v[u];
         }
     } else if (inner == 1) {
-        const double* __restrict__ base = sp.x + first * n;
+        const T* __restrict__ base = X + first * n;
         for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
-            double v[kBatch];
+            C v[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) v[u] = base[min(e0 + u * kTileThreads, count - 1)];
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(base[min(e0 + u * kTileThreads, count - 1)]);
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const uint32_t e = e0 + u * kTileThreads;
@@ -244,14 +256,14 @@ __global__ __launch_bounds__(kTileThreads) void solve_tile_kernel(SolveParams sp
             }
         }
     } else if (static_cast<int>(my_l) < nl) {
-        const double* __restrict__ line = sp.x + dst_base[my_l];
-        double* __restrict__ row = tile + my_l * stride;
+        const T* __restrict__ line = X + dst_base[my_l];
+        C* __restrict__ row = tile + my_l * stride;
         for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
-            double v[kBatch];
+            C v[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const uint32_t jj = min(e0 + u * kTileThreads, count - 1) >> lshift;
-                v[u] = line[static_cast<int64_t>(jj) * inner];
+                v[u] = static_cast<C>(line[static_cast<int64_t>(jj) * inner]);
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
@@ -262,14 +274,16 @@ __global__ __launch_bounds__(kTileThreads) void solve_tile_kernel(SolveParams sp
     }
     __syncthreads();
 
-    if (tid < nl) substitute<BW, CYC>(tile + tid * stride, 1, n, sp);
+    if (tid < nl) substitute<BW, CYC, C, C>(tile + tid * stride, 1, n, sp);
     __syncthreads();
 
     // ---- store ----
+    T* __restrict__ Y = sp.accum ? static_cast<T*>(sp.accum) : X;
+    const bool add = sp.accum != nullptr;
     if (inner == 1) {
-        double* __restrict__ base = sp.x + first * n;
+        T* __restrict__ base = Y + first * n;
         for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
-            double v[kBatch];
+            C v[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const uint32_t e = min(e0 + u * kTileThreads, count - 1);
@@ -278,24 +292,31 @@ __global__ __launch_bounds__(kTileThreads) void solve_tile_kernel(SolveParams sp
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u)
-                if (e0 + u * kTileThreads < count) base[e0 + u * kTileThreads] = v[u];
+                if (e0 + u * kTileThreads < count) {
+                    T* at = base + e0 + u * kTileThreads;
+                    *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
+                }
         }
     } else if (static_cast<int>(my_l) < nl) {
-        double* __restrict__ line = sp.x + dst_base[my_l];
-        const double* __restrict__ row = tile + my_l * stride;
+        T* __restrict__ line = Y + dst_base[my_l];
+        const C* __restrict__ row = tile + my_l * stride;
         for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
-            double v[kBatch];
+            C v[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) v[u] = row[min(e0 + u * kTileThreads, count - 1) >> lshift];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const uint32_t e = e0 + u * kTileThreads;
-                if (e < count) line[static_cast<int64_t>(e >> lshift) * inner] = v[u];
+                if (e < count) {
+                    T* at = line + static_cast<int64_t>(e >> lshift) * inner;
+                    *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
+                }
             }
         }
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void shift_copy_kernel(ShiftParams sp) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= sp.total) return;
@@ -309,69 +330,84 @@ __global__ __launch_bounds__(kBlock) void shift_copy_kernel(ShiftParams sp) {
         dst += w * mul;
         mul *= n;
     }
-    sp.dst[dst] = sp.src[i];
+    static_cast<T*>(sp.dst)[dst] = static_cast<T>(sp.src[i]);
 }
 
 // LDS a tiled workgroup may use: two workgroups per CU on gfx950
 // (160 KB per CU), never more than the device allows one workgroup.
 constexpr int64_t kTileLdsBudget = 80 * 1024;
 
-template <int BW>
+template <int BW, typename T, typename C>
 hipError_t launch_bw(bool cyclic, const SolveParams& p, hipStream_t s) {
     if (p.nlines == 0) return hipSuccess;
     if (p.lines_per_block > 0) {
         const int lpb = p.lines_per_block;
         if (lpb > kMaxTileLines || kTileThreads % lpb != 0) return hipErrorInvalidValue;
         const int64_t blocks = (p.nlines + lpb - 1) / lpb;
-        const size_t lds = static_cast<size_t>(lpb * tile_stride(p.n)) * sizeof(double);
+        const size_t lds = static_cast<size_t>(lpb * tile_stride(p.n)) * sizeof(C);
         if (cyclic)
-            hipLaunchKernelGGL((solve_tile_kernel<BW, true>), dim3(static_cast<unsigned>(blocks)), dim3(kTileThreads),
-                               lds, s, p);
+            hipLaunchKernelGGL((solve_tile_kernel<BW, true, T, C>), dim3(static_cast<unsigned>(blocks)),
+                               dim3(kTileThreads), lds, s, p);
         else
-            hipLaunchKernelGGL((solve_tile_kernel<BW, false>), dim3(static_cast<unsigned>(blocks)),
+            hipLaunchKernelGGL((solve_tile_kernel<BW, false, T, C>), dim3(static_cast<unsigned>(blocks)),
                                dim3(kTileThreads), lds, s, p);
         return hipGetLastError();
     }
-    if (p.src) return hipErrorInvalidValue;  // the HBM kernel solves in place only
+    if (p.src || p.accum) return hipErrorInvalidValue;  // the HBM kernel solves in place only
     const int64_t blocks = (p.nlines + kBlock - 1) / kBlock;
     if (cyclic)
-        hipLaunchKernelGGL((solve_kernel<BW, true>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+        hipLaunchKernelGGL((solve_kernel<BW, true, T, C>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s,
+                           p);
     else
-        hipLaunchKernelGGL((solve_kernel<BW, false>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+        hipLaunchKernelGGL((solve_kernel<BW, false, T, C>), dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0,
+                           s, p);
     return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_solve(int bw, bool cyclic, const SolveParams& p, hipStream_t s) {
+template <typename T, typename C>
+hipError_t launch_typed(int bw, bool cyclic, const SolveParams& p, hipStream_t s) {
     switch (bw) {
-        case 0: return launch_bw<0>(cyclic, p, s);
-        case 1: return launch_bw<1>(cyclic, p, s);
-        case 2: return launch_bw<2>(cyclic, p, s);
-        case 3: return launch_bw<3>(cyclic, p, s);
-        case 4: return launch_bw<4>(cyclic, p, s);
-        case 5: return launch_bw<5>(cyclic, p, s);
-        case 6: return launch_bw<6>(cyclic, p, s);
+        case 0: return launch_bw<0, T, C>(cyclic, p, s);
+        case 1: return launch_bw<1, T, C>(cyclic, p, s);
+        case 2: return launch_bw<2, T, C>(cyclic, p, s);
+        case 3: return launch_bw<3, T, C>(cyclic, p, s);
+        case 4: return launch_bw<4, T, C>(cyclic, p, s);
+        case 5: return launch_bw<5, T, C>(cyclic, p, s);
+        case 6: return launch_bw<6, T, C>(cyclic, p, s);
         default: return hipErrorNotSupported;
     }
 }
 
-int solve_tile_lines(int64_t n) {
+}  // namespace
+
+hipError_t launch_solve(int bw, bool cyclic, SolveTypes types, const SolveParams& p, hipStream_t s) {
+    switch (types) {
+        case SolveTypes::kF64: return launch_typed<double, double>(bw, cyclic, p, s);
+        case SolveTypes::kF32: return launch_typed<float, float>(bw, cyclic, p, s);
+        case SolveTypes::kF32StoreF64Compute: return launch_typed<float, double>(bw, cyclic, p, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int solve_tile_lines(int64_t n, int compute_bytes) {
     int dev = 0, max_lds = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         return 0;
     const int64_t budget = std::min<int64_t>(kTileLdsBudget, max_lds) - 2 * kMaxTileLines * sizeof(int64_t);
-    const int64_t row = tile_stride(n) * static_cast<int64_t>(sizeof(double));
+    const int64_t row = tile_stride(n) * static_cast<int64_t>(compute_bytes);
     int lpb = kMaxTileLines;
     while (lpb > 0 && lpb * row > budget) lpb /= 2;
     return lpb;
 }
 
-hipError_t launch_shift_copy(const ShiftParams& p, hipStream_t s) {
+hipError_t launch_shift_copy(const ShiftParams& p, bool dst_f32, hipStream_t s) {
     const int64_t blocks = (p.total + kBlock - 1) / kBlock;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(shift_copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+    if (dst_f32)
+        hipLaunchKernelGGL(shift_copy_kernel<float>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+    else
+        hipLaunchKernelGGL(shift_copy_kernel<double>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -55,6 +55,8 @@ typedef enum {
 #define INTP_EVAL_TIMED 8u        /* record HIP events around this call's
                                      grouping and its evaluation kernels,
                                      read back with intp_eval_timings()   */
+#define INTP_EVAL_OUT_F32 16u     /* `out` points to float storage: results
+                                     are stored as FP32 (refined splines) */
 
 typedef struct intp_ctx_s* intp_ctx;
 typedef struct intp_template_s* intp_template;
@@ -87,6 +89,8 @@ typedef struct {
     int64_t nknots[INTP_MAX_DIM];
     double range_lo[INTP_MAX_DIM];
     double range_hi[INTP_MAX_DIM];
+    int32_t ctrl_bytes;    /* 8: FP64 control points; 4: refined FP32      */
+    int32_t refine_sweeps; /* FP64-residual sweeps of a refined build, else -1 */
 } intp_spline_info;
 
 /* ---- errors -------------------------------------------------------- */
@@ -121,6 +125,20 @@ int intp_interpolate(intp_template t, const double* values, int values_mem, intp
  * x_ranges...) (Interpolation.hpp:92-107): template + interpolate.      */
 int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem,
                intp_spline* out, void* stream);
+
+/* Refined-FP32 form of intp_interpolate / intp_build: control points
+ * stored in FP32, solved with the FP32-rounded LU factors (FP64 factors
+ * for periodic dimensions of order >= 5), then `sweeps` (0..3) rounds of
+ * iterative refinement: the residual r = S f - (M_0 (x) ... (x) M_{D-1}) c
+ * is formed in FP64 from the unfactored collocation matrices and the FP64
+ * values, solved in FP32 and added to c.  Splines built this way are
+ * evaluated with FP32 weights and tables.  No reference counterpart: the
+ * reference solves in the value type T (InterpolationTemplate.hpp:448-580);
+ * this is its mixed-precision variant.                                  */
+int intp_interpolate_refined(intp_template t, const double* values, int values_mem, int32_t sweeps,
+                             intp_spline* out, void* stream);
+int intp_build_refined(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem,
+                       int32_t sweeps, intp_spline* out, void* stream);
 
 /* Replaces BSpline<T,D,O,U>(periodicity, ctrl_pts, knot_pairs...)
  * (BSpline.hpp:188-210): a spline from explicit knot vectors (host arrays,
