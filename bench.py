@@ -75,6 +75,78 @@ def rel_stats(got, want):
                 rms=float(math.sqrt(np.mean(d * d))))
 
 
+def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, sweeps: int = 1):
+    """The refined-FP32 form against the FP64 form on the same workload:
+    3D n^3 order-5, all clamped, nq value queries (FP32 results for the
+    refined form), plus the 1D periodic order-6 n=8192 spline whose solve
+    keeps FP64 factors.  Errors are the refined results against the FP64
+    form's (which match the FP64 reference to 1e-12): max|d|/max|ref| and
+    rms(d)/rms(ref)."""
+    from bsplineinterpolation_amd import datagen
+    w = datagen.workload_3d_256_quintic(n=n, nq=nq)
+    dev = torch.device("cuda:0")
+    f_d = torch.from_numpy(w["f"]).to(dev)
+    q_d = torch.from_numpy(np.ascontiguousarray(w["queries"].T)).to(dev)
+    outs = {"f64": torch.empty((1, nq), dtype=torch.float64, device=dev),
+            "f32_refined": torch.empty((1, nq), dtype=torch.float32, device=dev)}
+    stream = torch.cuda.current_stream()
+    res = {}
+    for form in ("f64", "f32_refined"):
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record(stream)
+            s = amd.InterpolationFunction(5, f_d, periodic=w["periodic"], ranges=w["ranges"], precision=form,
+                                          sweeps=sweeps)
+            if ev is not None:
+                ev[1].record(stream)
+            s.eval(q_d, w["orders"], out=outs[form], timed=ev is not None)
+            if ev is not None:
+                ev[2].record(stream)
+            return s
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        amd.eval_timings()
+        events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(events[i])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        tim = amd.eval_timings()
+        res[form] = {"ms_per_step": ms, "value": nq / (ms * 1e-3),
+                     "build_ms": float(np.mean([e[0].elapsed_time(e[1]) for e in events])),
+                     "eval_ms": float(np.mean([e[1].elapsed_time(e[2]) for e in events])),
+                     "eval_grouping_ms": float(np.mean([t[0] for t in tim])),
+                     "eval_kernel_ms": float(np.mean([t[1] for t in tim]))}
+    ref = outs["f64"].double()
+    d = outs["f32_refined"].double() - ref
+    err = {"max": float(d.abs().max() / ref.abs().max()),
+           "rms": float(torch.sqrt((d * d).mean() / (ref * ref).mean()))}
+    # 1D periodic order 6, n = 8192 (FP64 factors in the refined solve)
+    w1 = datagen.workload_1d_periodic_sextic()
+    q1 = torch.from_numpy(np.ascontiguousarray(w1["queries"])).to(dev)
+    s64 = amd.InterpolationFunction(6, w1["f"], periodic=w1["periodic"], ranges=w1["ranges"])
+    s32 = amd.InterpolationFunction(6, w1["f"], periodic=w1["periodic"], ranges=w1["ranges"],
+                                    precision="f32_refined", sweeps=sweeps)
+    r64 = s64.eval(q1)
+    d1 = s32.eval(q1) - r64
+    err1 = {"max": float(d1.abs().max() / r64.abs().max()), "rms": float(torch.sqrt((d1 * d1).mean() / (r64 * r64).mean()))}
+    b = res["f32_refined"]
+    N = n ** 3
+    # algorithmic bytes (S sweeps): FP32 solves (S+1) 2 D N 4, FP64 residuals
+    # S (2 D N 8 + 8 N), FP32 updates S N 4; evaluation nq (24 + 4) + N 4
+    b_build = (sweeps + 1) * 2 * 3 * N * 4 + sweeps * (2 * 3 * N * 8 + 8 * N) + sweeps * N * 4
+    b_eval = nq * (24 + 4) + N * 4
+    return {"workload": "3D %d^3 order-5 clamped, %d queries (value); FP32 control points, FP32 results" % (n, nq),
+            "sweeps": sweeps, "refined_f32": res["f32_refined"], "f64_form": res["f64"],
+            "speedup_vs_f64_form": res["f64"]["ms_per_step"] / b["ms_per_step"],
+            "error_vs_f64": err,
+            "build_gbs": b_build / (b["build_ms"] * 1e-3) / 1e9, "build_algorithmic_bytes": b_build,
+            "eval_kernel_gbs": b_eval / (b["eval_kernel_ms"] * 1e-3) / 1e9, "eval_algorithmic_bytes": b_eval,
+            "periodic_1d_order6_n8192": {"error_vs_f64": err1, "note": "periodic order >= 5: FP64 factors"}}
+
+
 def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
     """Reference CPU path, bounded sample.  Returns (dict, error stats)."""
     import oracle
@@ -114,6 +186,9 @@ def main():
     ap.add_argument("--queries", type=int, default=1 << 26)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=1 << 22)
+    ap.add_argument("--no-forms", action="store_true", help="skip the refined-FP32 form measurement")
+    ap.add_argument("--only-forms", action="store_true",
+                    help="measure only the refined-FP32 form (profiling runs); prints its object")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="copy 256 MiB once before timing (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -127,6 +202,9 @@ def main():
     from bsplineinterpolation_amd import datagen
 
     n, nq = args.n, args.queries
+    if args.only_forms:
+        print(json.dumps(measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)))
+        return
     print("bench: generating inputs (%d^3 mesh, %d queries)" % (n, nq), file=sys.stderr, flush=True)
     w = datagen.workload_3d_256_cubic(n=n, nq=nq)
     w["nq"] = nq
@@ -226,6 +304,12 @@ def main():
                      "traffic": profiled_traffic(EVAL_KERNEL) if (n, nq) == (256, 1 << 26) else None,
                      "traffic_source": TRAFFIC_PROFILE},
     }
+
+    if not args.no_forms:
+        print("bench: refined-FP32 form (3D order 5)", file=sys.stderr, flush=True)
+        del keep
+        keep = None
+        result["forms"] = measure_refined_form(amd, torch, max(3, args.steps // 2), 2, n, nq)
 
     if not args.no_cpu:
         m = min(args.cpu_sample, nq)

@@ -553,7 +553,7 @@ constexpr int64_t kGroupMinPoints = int64_t{1} << 20;
 
 // Bricks of B^D stencil starts, B a power of two: the largest B whose
 // (B + order)^D patch of table elements (elem_bytes each) stays within
-// 16 KB (the LDS patch of the grouped
+// kPatchBytes (the LDS patch of the grouped
 // evaluation kernel), coarsened until the brick key fits 16 bits (then
 // the patch no longer fits and the grouped kernel reads HBM only).
 BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
@@ -566,7 +566,7 @@ BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
         for (int d = 0; d < g.dim; ++d) v *= b + g.order;
         return v;
     };
-    while (patch(2 * B) * elem_bytes <= 16384) B *= 2;
+    while (patch(2 * B) * elem_bytes <= kPatchBytes) B *= 2;
     for (;;) {
         int64_t nbin = 1;
         for (int d = 0; d < g.dim; ++d) nbin *= (g.setup[d].n + B - 1) / B;
@@ -580,7 +580,7 @@ BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
         bp.bricks.brick_shift[d] = shift;
         bp.bricks.nbrick[d] = static_cast<int32_t>((g.setup[d].n + B - 1) / B);
     }
-    *patch_volume = (patch(B) * elem_bytes <= 16384 && B + g.order <= 64) ? patch(B) : 0;
+    *patch_volume = (patch(B) * elem_bytes <= kPatchBytes && B + g.order <= 64) ? patch(B) : 0;
     return bp;
 }
 
@@ -913,10 +913,15 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         else if (sum == 1 && unit) slot = which + 1;
         if (slot >= 0 && p.slot[slot] < 0) {
             p.slot[slot] = o;
-            any_grad = true;
+            any_grad = any_grad || slot > 0;
         } else {
             single.push_back(o);
         }
+    }
+    if (!any_grad && p.slot[0] >= 0) {
+        // the value alone: the single-output kernel contracts one set of weights
+        single.insert(single.begin(), p.slot[0]);
+        p.slot[0] = -1;
     }
     {
         bool packed = !out_f32 && os_out == 1 && os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;

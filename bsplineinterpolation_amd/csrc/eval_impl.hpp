@@ -219,6 +219,15 @@ __device__ __forceinline__ void poly_weights(const double* __restrict__ P, doubl
             dd.w[j] = static_cast<W>(v);
             dd.dw[j] = static_cast<W>(K >= 1 ? g * inv_dx : 0.0);
         }
+    } else if (r == 0) {
+        // the value: Horner
+#pragma unroll
+        for (int j = 0; j <= K; ++j) {
+            double v = P[j * NW + K];
+#pragma unroll
+            for (int q = K - 1; q >= 0; --q) v = v * u + P[j * NW + q];
+            dd.w[j] = static_cast<W>(v);
+        }
     } else {
         // r-th derivative: sum_q q!/(q-r)! P_q u^(q-r), times dx^-r
         double scale = 1.0;
@@ -502,7 +511,6 @@ __global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
 
 // ---- grouped evaluation with the brick patch in LDS ---------------------
 
-constexpr int kPatchBytes = 16384;  // one brick patch in LDS
 template <typename T>
 constexpr int patch_max() {
     return kPatchBytes / static_cast<int>(sizeof(T));

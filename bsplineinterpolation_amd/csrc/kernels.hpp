@@ -36,6 +36,9 @@ struct DimParams {
 // of a point is its brick index in C order.  Brick b's stencils read only
 // control points [b*B, b*B + B - 1 + order] (B = 1 << brick_shift), wrapped
 // on periodic dimensions.
+// LDS bytes of one brick patch in the grouped evaluation kernels.
+constexpr int kPatchBytes = 24576;
+
 struct BrickGrid {
     int32_t brick_shift[kMaxDim];
     int32_t nbrick[kMaxDim];

@@ -114,3 +114,32 @@ def workload_3d_256_cubic(seed: int = SEED, n: int = 256, nq: int = 1 << 26, nq_
             q[d] = lo[d] + (hi[d] - lo[d]) * uniform(seed, STREAM_QUERY, m, offset=d * nq)
     return dict(order=3, f=f, periodic=[True, False, False], ranges=[(0.0, 2 * math.pi), (0.0, 1.0), (0.0, 1.0)],
                 queries=q, orders=[[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+
+
+def workload_3d_256_quintic(seed: int = SEED, n: int = 256, nq: int = 1 << 26):
+    """3D order-5 on n^3 over [0,2pi] x [0,1] x [0,1], all dimensions
+    clamped (x_i = 2 pi i / (n-1)), the same value formula and noise as
+    workload_3d_256_cubic; nq uniform queries, value only.  The
+    refined-FP32 form is measured on it."""
+    x = grid_axis(0.0, 2 * math.pi, n, False)
+    y = grid_axis(0.0, 1.0, n, False)
+    z = grid_axis(0.0, 1.0, n, False)
+    f = (np.sin(x)[:, None, None] * np.exp(-((y - 0.5) ** 2) / 0.05)[None, :, None]
+         * np.cos(4 * math.pi * z)[None, None, :])
+    f = f + normal(seed, STREAM_NOISE, f.size, 1e-3).reshape(f.shape)
+    lo, hi = [0.0, 0.0, 0.0], [2 * math.pi, 1.0, 1.0]
+    q = uniform(seed, STREAM_QUERY, 3 * nq).reshape(3, nq)
+    for d in range(3):
+        q[d] = lo[d] + (hi[d] - lo[d]) * q[d]
+    return dict(order=5, f=f, periodic=[False, False, False], ranges=[(0.0, 2 * math.pi), (0.0, 1.0), (0.0, 1.0)],
+                queries=q, orders=[[0, 0, 0]])
+
+
+def workload_1d_periodic_sextic(seed: int = SEED, n: int = 8192, nq: int = 1 << 20):
+    """1D periodic order 6, n values on [0, 2pi): sin x + 0.3 cos 5x +
+    N(0, 1e-3^2), nq uniform queries (value).  Its periodic order-6 solve
+    keeps FP64 factors in the refined form."""
+    x = grid_axis(0.0, 2 * math.pi, n, True)
+    f = np.sin(x) + 0.3 * np.cos(5 * x) + normal(seed, STREAM_NOISE, n, 1e-3)
+    q = 2 * math.pi * uniform(seed, STREAM_QUERY, nq).reshape(1, nq)
+    return dict(order=6, f=f, periodic=[True], ranges=[(0.0, 2 * math.pi)], queries=q, orders=[[0]])
