@@ -32,7 +32,7 @@ namespace intp_amd {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kEvalBlock = 256;
 
 #ifndef INTP_OUTER_UNROLL_MAX_K
 #define INTP_OUTER_UNROLL_MAX_K 2  // fully unroll the outer stencil loop up to this order
@@ -466,8 +466,8 @@ __device__ __forceinline__ void general_point(const EvalParams& p, int64_t pt) {
 }
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
-__global__ __launch_bounds__(kBlock) void eval_general_kernel(EvalParams p) {
-    const int64_t pt = block_index() * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kEvalBlock) void eval_general_kernel(EvalParams p) {
+    const int64_t pt = block_index() * kEvalBlock + threadIdx.x;
     if (pt < p.npts) general_point<D, K, GRAD, LASTC, T>(p, pt);
 }
 
@@ -495,17 +495,17 @@ __device__ __forceinline__ void uniform_point(const EvalParams& p, const double 
 }
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
-__global__ __launch_bounds__(kBlock) void eval_uniform_kernel(EvalParams p) {
+__global__ __launch_bounds__(kEvalBlock) void eval_uniform_kernel(EvalParams p) {
     constexpr int kClassMax = 1 + 2 * K;
     constexpr int kCoef = (K + 1) * (K + 1);
     __shared__ double s_poly[D][kClassMax * kCoef];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int n = p.dim[d].nclass * kCoef;
-        for (int i = threadIdx.x; i < n; i += kBlock) s_poly[d][i] = p.dim[d].poly[i];
+        for (int i = threadIdx.x; i < n; i += kEvalBlock) s_poly[d][i] = p.dim[d].poly[i];
     }
     __syncthreads();
-    const int64_t pt = block_index() * kBlock + threadIdx.x;
+    const int64_t pt = block_index() * kEvalBlock + threadIdx.x;
     if (pt < p.npts) uniform_point<D, K, GRAD, LASTC, T>(p, s_poly, pt);
 }
 
@@ -570,7 +570,7 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
 // point, (B + K)^D control points (wrapped on periodic dimensions, zero
 // past the end of clamped ones, where no stencil reaches), then evaluates.
 template <int D, int K, bool GRAD, bool LASTC, typename T>
-__global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
+__global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
     constexpr int kClassMax = 1 + 2 * K;
     constexpr int kCoef = (K + 1) * (K + 1);
     constexpr int kPatchMax = patch_max<T>();
@@ -580,10 +580,10 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int n = p.dim[d].nclass * kCoef;
-        for (int i = threadIdx.x; i < n; i += kBlock) s_poly[d][i] = p.dim[d].poly[i];
+        for (int i = threadIdx.x; i < n; i += kEvalBlock) s_poly[d][i] = p.dim[d].poly[i];
     }
-    const int64_t first = block_index() * kBlock;
-    const int64_t pivot = min(first + kBlock / 2, p.npts - 1);
+    const int64_t first = block_index() * kEvalBlock;
+    const int64_t pivot = min(first + kEvalBlock / 2, p.npts - 1);
     uint32_t rem = p.key[pivot * p.perm_stride];
     int bstart[D], bsize[D], psize[D];
     uint32_t lstride[D];
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
     // dimension 0 with a running index: no divisions in the loop.
     if (vol > 0) {
         constexpr int kInner = D == 3 ? 16 : 64;     // threads along the last dimension
-        constexpr int kGroups = kBlock / (D == 3 ? 256 : 64);  // D = 2: 4 groups split dimension 0
+        constexpr int kGroups = kEvalBlock / (D == 3 ? 256 : 64);  // D = 2: 4 groups split dimension 0
         const int t = threadIdx.x;
         const int i_last = t % kInner;
         const int i_mid = D == 3 ? t / kInner : 0;
@@ -676,10 +676,10 @@ __global__ __launch_bounds__(kBlock) void eval_patch_kernel(EvalParams p) {
 
 template <void (*KF)(EvalParams)>
 hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
-    const int64_t blocks = (p.npts + kBlock - 1) / kBlock;
+    const int64_t blocks = (p.npts + kEvalBlock - 1) / kEvalBlock;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(KF, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, p);
+    hipLaunchKernelGGL(KF, dim3(static_cast<unsigned>(blocks)), dim3(kEvalBlock), 0, s, p);
     return hipGetLastError();
 }
 

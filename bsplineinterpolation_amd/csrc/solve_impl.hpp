@@ -1,0 +1,314 @@
+// solve_impl.hpp -- control-point solve on gfx950 (device code shared by
+// solve.hip and the batched kernels of batch.hip): every grid line of one axis
+// is an independent band (or cyclic band) system sharing one LU
+// factorization (InterpolationTemplate.hpp:448-580).  One lane solves one
+// line.
+//
+// Main kernel (solve_tile_kernel): a workgroup copies the lines of up to
+// 64 neighbouring lanes into LDS with coalesced bulk loads (runs of one
+// line for the contiguous last axis, runs across lines for the strided
+// axes), each lane substitutes its line in LDS, and the workgroup writes
+// the tile back the same way.  The first pass reads the caller's values
+// with the periodic index shift applied (InterpolationTemplate.hpp:452-462),
+// so no separate copy precedes it.  Lines too long for LDS fall back to
+// solve_kernel, which substitutes in place in HBM.
+//
+// The substitution is the reference's (BandLU.hpp:121-143, 216-259)
+// reordered by rows: each entry receives the same updates in the same
+// order, the last `bw` values ride in registers, and the cyclic strips are
+// register accumulators (forward) and register-held corner values
+// (backward).  Built without FMA contraction and with a true division by
+// the pivot, the result is bit-identical to the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace intp_amd {
+
+namespace {
+
+constexpr int kSolveBlock = 256;
+constexpr int kTileThreads = 256;
+constexpr int kMaxTileLines = 64;
+
+// Elements a lane moves per batch: their loads (values and factor
+// coefficients) are issued together, then substituted in order, then
+// stored.
+constexpr int kBatch = 8;
+
+template <int BW, bool CYC, typename C>
+struct LineState {
+    C hist[BW + 1];    // forward: hist[m-1] = y[j-m]
+    C acc[BW + 1];     // forward: rows n-BW+r of a cyclic matrix, strip-updated
+    C nxt[BW + 1];     // backward: nxt[m-1] = x[j+m]
+    C corner[BW + 1];  // backward: x[n-BW+c]
+};
+
+// Forward substitution of rows j0 .. j0+B-1 (unit lower triangle, plus
+// the bottom strip of a cyclic matrix).
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void forward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
+                                             const SolveParams& sp, LineState<BW, CYC, C>& st) {
+#pragma clang fp contract(off)
+    const C* __restrict__ lower = static_cast<const C*>(sp.lower);
+    const C* __restrict__ bottom = static_cast<const C*>(sp.bottom);
+    C xb[B];
+    C lc[B][BW + 1];
+    C sc[B][BW + 1];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        xb[u] = static_cast<C>(x[(j0 + u) * step]);
+#pragma unroll
+        for (int m = 0; m < BW; ++m) lc[u][m] = lower[(j0 + u) * BW + m];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int r = 0; r < BW; ++r) sc[u][r] = bottom[r * n + j0 + u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int64_t j = j0 + u;
+        C v = xb[u];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int r = 0; r < BW; ++r)
+                if (j == n - BW + r) v = st.acc[r];
+        }
+#pragma unroll
+        for (int m = BW; m >= 1; --m) v = v - lc[u][m - 1] * st.hist[m - 1];
+        xb[u] = v;
+#pragma unroll
+        for (int m = BW; m >= 2; --m) st.hist[m - 1] = st.hist[m - 2];
+        if (BW > 0) st.hist[0] = v;
+        if constexpr (CYC) {
+#pragma unroll
+            for (int r = 0; r < BW; ++r)
+                if (j < n - BW + r - BW) st.acc[r] = st.acc[r] - sc[u][r] * v;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) x[(j0 + u) * step] = static_cast<P>(xb[u]);
+}
+
+// Backward substitution of rows j0, j0-1, .., j0-B+1 (upper triangle,
+// plus the right strip of a cyclic matrix).
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void backward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
+                                              const SolveParams& sp, LineState<BW, CYC, C>& st) {
+#pragma clang fp contract(off)
+    const C* __restrict__ upper = static_cast<const C*>(sp.upper);
+    const C* __restrict__ diag = static_cast<const C*>(sp.diag);
+    const C* __restrict__ right = static_cast<const C*>(sp.right);
+    C xb[B];
+    C uc[B][BW + 1];
+    C rc[B][BW + 1];
+    C dg[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int64_t j = j0 - u;
+        xb[u] = static_cast<C>(x[j * step]);
+#pragma unroll
+        for (int m = 0; m < BW; ++m) uc[u][m] = upper[j * BW + m];
+        dg[u] = diag[j];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int c = 0; c < BW; ++c) rc[u][c] = right[c * n + j];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int64_t j = j0 - u;
+        C v = xb[u];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int c = BW - 1; c >= 0; --c)
+                if (j < n - BW + c - BW) v = v - rc[u][c] * st.corner[c];
+        }
+#pragma unroll
+        for (int m = BW; m >= 1; --m) v = v - uc[u][m - 1] * st.nxt[m - 1];
+        v = v / dg[u];
+        xb[u] = v;
+#pragma unroll
+        for (int m = BW; m >= 2; --m) st.nxt[m - 1] = st.nxt[m - 2];
+        if (BW > 0) st.nxt[0] = v;
+        if constexpr (CYC) {
+#pragma unroll
+            for (int c = 0; c < BW; ++c)
+                if (j == n - BW + c) st.corner[c] = v;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) x[(j0 - u) * step] = static_cast<P>(xb[u]);
+}
+
+// Solves one line x[0], x[step], ..., x[(n-1)*step] in place, computing
+// in C (elements of x have type P).
+template <int BW, bool CYC, typename C, typename P>
+__device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
+    LineState<BW, CYC, C> st;
+#pragma unroll
+    for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = C(0);
+    if constexpr (CYC) {
+#pragma unroll
+        for (int r = 0; r < BW; ++r) st.acc[r] = static_cast<C>(x[(n - BW + r) * step]);
+    }
+    int64_t j = 0;
+    for (; j + kBatch <= n; j += kBatch) forward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
+    for (; j < n; ++j) forward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
+    j = n - 1;
+    for (; j - kBatch + 1 >= 0; j -= kBatch) backward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
+    for (; j >= 0; --j) backward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
+}
+
+// Fallback: one lane per line, substituting in HBM (storage T, compute C;
+// with T narrower than C the forward results are rounded to T between
+// the two sweeps).
+
+// Odd LDS row stride (in elements), so that lanes reading the same column
+// of their rows hit distinct banks.
+__host__ __device__ __forceinline__ int64_t tile_stride(int64_t n) { return n | 1; }
+
+// One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS in
+// the compute type C; the table holds T.  With sp.accum set, the solved
+// lines are added into accum instead of being stored to x.
+template <int BW, bool CYC, typename T, typename C>
+__device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first, unsigned char* tile_bytes,
+                                           int64_t* dst_base, int64_t* src_base) {
+    C* tile = reinterpret_cast<C*>(tile_bytes);
+    T* __restrict__ X = static_cast<T*>(sp.x);
+    const int lpb = sp.lines_per_block;
+    const int64_t n = sp.n, inner = sp.inner, stride = tile_stride(n);
+    const int nl = static_cast<int>(min(static_cast<int64_t>(lpb), sp.nlines - first));
+    const int tid = threadIdx.x;
+    if (tid < nl) {
+        const int64_t line = first + tid;
+        const int64_t o = line / inner, ii = line - o * inner;
+        dst_base[tid] = o * n * inner + ii;
+        if (sp.src) {
+            // inner == 1: o runs over the outer dims in C order
+            int64_t rem = o, off = 0, mul = n;
+            for (int d = sp.outer_dims - 1; d >= 0; --d) {
+                const int64_t nd = sp.outer_n[d];
+                const int64_t id = rem % nd;
+                rem /= nd;
+                int64_t w = id - sp.outer_shift[d];
+                if (w < 0) w += nd;
+                off += w * mul;
+                mul *= nd;
+            }
+            src_base[tid] = off;
+        }
+    }
+    __syncthreads();
+
+    // ---- load ----
+    // Contiguous lines: tile element e is (line e / n, element e % n).
+    // Strided lines: (line e % lpb, element e / lpb); as 256 % lpb == 0 a
+    // thread keeps one line.
+    const uint32_t un = static_cast<uint32_t>(n);
+    const int lshift = __builtin_ctz(static_cast<unsigned>(lpb));
+    const uint32_t count = static_cast<uint32_t>(inner == 1 ? nl * n : lpb * n);
+    const uint32_t my_l = static_cast<uint32_t>(tid) & static_cast<uint32_t>(lpb - 1);
+    if (sp.src) {
+        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+            int64_t from[kBatch];
+            uint32_t at[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t e = min(e0 + u * kTileThreads, count - 1);
+                const uint32_t l = e / un, jj = e - l * un;
+                int64_t js = static_cast<int64_t>(jj) - sp.shift_last;
+                if (js < 0) js += n;
+                from[u] = src_base[l] + js;
+                at[u] = l * static_cast<uint32_t>(stride) + jj;
+            }
+            C v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(sp.src[from[u]]);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (e0 + u * kTileThreads < count) tile[at[u]] = v[u];
+        }
+    } else if (inner == 1) {
+        const T* __restrict__ base = X + first * n;
+        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+            C v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(base[min(e0 + u * kTileThreads, count - 1)]);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t e = e0 + u * kTileThreads;
+                if (e < count) {
+                    const uint32_t l = e / un, jj = e - l * un;
+                    tile[l * static_cast<uint32_t>(stride) + jj] = v[u];
+                }
+            }
+        }
+    } else if (static_cast<int>(my_l) < nl) {
+        const T* __restrict__ line = X + dst_base[my_l];
+        C* __restrict__ row = tile + my_l * stride;
+        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+            C v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t jj = min(e0 + u * kTileThreads, count - 1) >> lshift;
+                v[u] = static_cast<C>(line[static_cast<int64_t>(jj) * inner]);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t e = e0 + u * kTileThreads;
+                if (e < count) row[e >> lshift] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    if (tid < nl) substitute<BW, CYC, C, C>(tile + tid * stride, 1, n, sp);
+    __syncthreads();
+
+    // ---- store ----
+    T* __restrict__ Y = sp.accum ? static_cast<T*>(sp.accum) : X;
+    const bool add = sp.accum != nullptr;
+    if (inner == 1) {
+        T* __restrict__ base = Y + first * n;
+        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+            C v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t e = min(e0 + u * kTileThreads, count - 1);
+                const uint32_t l = e / un, jj = e - l * un;
+                v[u] = tile[l * static_cast<uint32_t>(stride) + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u)
+                if (e0 + u * kTileThreads < count) {
+                    T* at = base + e0 + u * kTileThreads;
+                    *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
+                }
+        }
+    } else if (static_cast<int>(my_l) < nl) {
+        T* __restrict__ line = Y + dst_base[my_l];
+        const C* __restrict__ row = tile + my_l * stride;
+        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+            C v[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) v[u] = row[min(e0 + u * kTileThreads, count - 1) >> lshift];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const uint32_t e = e0 + u * kTileThreads;
+                if (e < count) {
+                    T* at = line + static_cast<int64_t>(e >> lshift) * inner;
+                    *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+}  // namespace intp_amd
