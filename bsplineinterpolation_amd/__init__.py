@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib
 from ._lib import DomainError, IntpError, check  # noqa: F401
 
-__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "DomainError", "IntpError",
+__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "Batch", "DomainError", "IntpError",
            "eval_timings"]
 
 _dp = C.POINTER(C.c_double)
@@ -324,3 +324,80 @@ class InterpolationFunctionTemplate:
         else:
             check(lib.intp_interpolate(self.h, vp, mem, C.byref(h), st))
         return InterpolationFunction._wrap(h, self.ctx)
+
+
+class Batch:
+    """Many independent FP64 splines built and evaluated together
+    (intp_batch_*).  ``specs``: one dict per spline with ``order``,
+    ``shape`` and optionally ``periodic``, ``ranges``, ``coords`` (as for
+    InterpolationFunction).  Values are the splines' C-order meshes
+    concatenated (``offsets`` gives each start); points are [npts, 3]
+    (x, y, z; unused dims ignored), spline s owning
+    ``point_offsets[s]:point_offsets[s+1]``."""
+
+    def __init__(self, specs, ctx: Optional[Context] = None):
+        lib = _lib.load()
+        self.ctx = ctx or Context.default()
+        self.count = len(specs)
+        descs = (_lib.GridDesc * self.count)()
+        keep = []
+        for i, sp in enumerate(specs):
+            d, k = _grid_desc(sp["order"], tuple(sp["shape"]), sp.get("periodic"), sp.get("ranges"), sp.get("coords"))
+            descs[i] = d
+            keep.append(k)
+        h = C.c_void_p()
+        check(lib.intp_batch_create(self.ctx.h, self.count, descs, C.byref(h)))
+        self.h = h
+        del keep
+        total = C.c_int64()
+        self.offsets = np.zeros(self.count + 1, np.int64)
+        check(lib.intp_batch_info(h, C.byref(total), self.offsets.ctypes.data_as(C.POINTER(C.c_int64))))
+        self.total = int(total.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.intp_batch_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _offsets(point_offsets):
+        po = np.ascontiguousarray(np.asarray(point_offsets, dtype=np.int64))
+        return po, po.ctypes.data_as(C.POINTER(C.c_int64))
+
+    def build(self, values, stream=None):
+        st = _stream_handle(stream) if stream is not None else _current_stream()
+        check(_lib.load().intp_batch_build(self.h, C.cast(values.data_ptr(), _dp), st))
+
+    def eval(self, coords, point_offsets, out=None, stream=None):
+        import torch
+        po, pp = self._offsets(point_offsets)
+        if out is None:
+            out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
+        st = _stream_handle(stream) if stream is not None else _current_stream()
+        check(_lib.load().intp_batch_eval(self.h, C.cast(coords.data_ptr(), _dp), pp, C.cast(out.data_ptr(), _dp), st))
+        return out
+
+    def run(self, values, coords, point_offsets, out=None, graph: bool = False, autotune: bool = False,
+            stream=None):
+        """Build + eval (device tensors); ``graph`` replays one captured
+        hipGraph while buffers and layout stay the same."""
+        import torch
+        po, pp = self._offsets(point_offsets)
+        if out is None:
+            out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
+        flags = (_lib.INTP_BATCH_GRAPH if graph else 0) | (_lib.INTP_BATCH_AUTOTUNE if autotune else 0)
+        st = _stream_handle(stream) if stream is not None else _current_stream()
+        check(_lib.load().intp_batch_run(self.h, C.cast(values.data_ptr(), _dp), C.cast(coords.data_ptr(), _dp), pp,
+                                         C.cast(out.data_ptr(), _dp), flags, st))
+        return out
+
+    def tuning(self) -> str:
+        buf = C.create_string_buffer(4096)
+        check(_lib.load().intp_batch_tuning(self.h, buf, 4096))
+        return buf.value.decode()
+
+    def control_points(self, s: int) -> np.ndarray:
+        n = int(self.offsets[s + 1] - self.offsets[s])
+        out = np.empty(n)
+        check(_lib.load().intp_batch_download_ctrl(self.h, int(s), out.ctypes.data_as(_dp), n))
+        return out

@@ -28,6 +28,8 @@ INTP_EVAL_HOST_MEM = 2
 INTP_EVAL_UNGROUPED = 4
 INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
+INTP_BATCH_GRAPH = 1
+INTP_BATCH_AUTOTUNE = 2
 
 # every symbol include/intp_amd.h declares
 EXPORTS = (
@@ -36,6 +38,8 @@ EXPORTS = (
     "intp_spline_from_knots", "intp_spline_retain", "intp_spline_release", "intp_spline_get_info",
     "intp_spline_download_ctrl", "intp_spline_download_knots", "intp_eval", "intp_eval_timings",
     "intp_interpolate_refined", "intp_build_refined",
+    "intp_batch_create", "intp_batch_destroy", "intp_batch_info", "intp_batch_build", "intp_batch_eval",
+    "intp_batch_run", "intp_batch_tuning", "intp_batch_download_ctrl",
 )
 
 
@@ -94,6 +98,14 @@ def load(path: str = LIB_PATH):
     lib.intp_eval_timings.argtypes = [dp, dp, C.c_int64, i64p]
     lib.intp_interpolate_refined.argtypes = [vp, dp, C.c_int, C.c_int32, C.POINTER(vp), vp]
     lib.intp_build_refined.argtypes = [vp, C.POINTER(GridDesc), dp, C.c_int, C.c_int32, C.POINTER(vp), vp]
+    lib.intp_batch_create.argtypes = [vp, C.c_int32, C.POINTER(GridDesc), C.POINTER(vp)]
+    lib.intp_batch_destroy.argtypes = [vp]
+    lib.intp_batch_info.argtypes = [vp, i64p, i64p]
+    lib.intp_batch_build.argtypes = [vp, dp, vp]
+    lib.intp_batch_eval.argtypes = [vp, dp, i64p, dp, vp]
+    lib.intp_batch_run.argtypes = [vp, dp, dp, i64p, dp, C.c_uint32, vp]
+    lib.intp_batch_tuning.argtypes = [vp, C.c_char_p, C.c_int64]
+    lib.intp_batch_download_ctrl.argtypes = [vp, C.c_int32, dp, C.c_int64]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

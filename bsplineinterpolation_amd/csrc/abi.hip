@@ -17,6 +17,7 @@
 
 #include "../../include/intp_amd.h"
 #include "kernels.hpp"
+#include "grid.hpp"
 #include "runtime.hpp"
 #include "setup.hpp"
 
@@ -161,9 +162,6 @@ hipError_t timing_event(int device, hipEvent_t* out) {
 
 }  // namespace
 
-struct intp_ctx_s {
-    int device;
-};
 
 struct intp_template_s {
     std::shared_ptr<GridData> grid;
@@ -372,31 +370,9 @@ int refine_into(const GridData& g, const double* values, float* ctrl, int sweeps
 }
 
 DimParams dim_params(const GridData& g, int d) {
-    const DimSetup& s = g.setup[d];
-    DimParams p{};
-    p.knots = g.knots[d];
-    p.recip = g.recip[d];
-    p.n = s.n;
     int64_t stride = 1;
     for (int e = d + 1; e < g.dim; ++e) stride *= g.setup[e].n;
-    p.stride = stride;
-    p.nt = static_cast<int32_t>(s.knots.size());
-    p.periodic = s.periodic ? 1 : 0;
-    p.uniform = s.uniform ? 1 : 0;
-    p.lo = s.lo;
-    p.hi = s.hi;
-    p.period = s.hi - s.lo;
-    p.inv_dx = s.uniform ? 1.0 / s.dx : 0.0;
-    p.shift = s.periodic ? 1.0 : 0.5 * static_cast<double>(g.order + 1);
-    p.xmin = s.xmin;
-    p.xmax = s.xmax;
-    p.dx = s.dx;
-    p.half = s.half;
-    p.poly = g.poly[d];
-    p.nleft = s.nleft;
-    p.nright = s.nright;
-    p.nclass = s.nclass;
-    return p;
+    return make_dim_params(g.setup[d], g.order, stride, g.knots[d], g.recip[d], g.poly[d]);
 }
 
 // Batches at least this large are grouped by brick before evaluation.

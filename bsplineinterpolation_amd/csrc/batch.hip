@@ -1,0 +1,633 @@
+// batch.hip -- the batched form (intp_batch_*): thousands of independent
+// FP64 splines of mixed dimension, order, size and periodicity, built and
+// evaluated by a fixed set of persistent launches, optionally captured
+// into one hipGraph and replayed, with tile shapes picked by an autotuner
+// that times candidates on the batch itself.
+//
+// Per spline the arithmetic is the single-spline path's (knots and LU on
+// the host, InterpolationTemplate.hpp:60-79; line solves last axis first,
+// :448-580; evaluation Interpolation.hpp:132-169), so every control point
+// equals the one intp_build computes for the same grid and values.
+//
+// Launch structure of one build + eval:
+//   solve pass p (p = 0..2): the axis D-1-p of every spline with D > p,
+//     one persistent launch per (band width, cyclic) class over tiles of
+//     lines (batch_solve_kernel), the first pass reading the values with
+//     the periodic shift;
+//   eval: one persistent launch per (dimension, order) class over runs of
+//     points of one spline (batch_eval_kernel, batch_eval_d*.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/intp_amd.h"
+#include "batch_impl.hpp"
+#include "grid.hpp"
+#include "runtime.hpp"
+#include "solve_impl.hpp"
+
+namespace intp_amd {
+
+namespace {
+
+// Persistent: workgroup b solves items b, b + gridDim.x, ... (tiles of
+// lines of one spline along one axis, all of band width BW).
+template <int BW, bool CYC>
+__global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSpline* specs,
+                                                                   const BatchSolveItem* items, int32_t nitems,
+                                                                   const double* values, double* ctrl,
+                                                                   int32_t first_pass) {
+    extern __shared__ __align__(16) unsigned char tile_bytes[];
+    __shared__ int64_t dst_base[kMaxTileLines];
+    __shared__ int64_t src_base[kMaxTileLines];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchSolveItem w = items[it];
+        const BatchSpline& b = specs[w.spline];
+        const int a = w.axis;
+        SolveParams sp{};
+        sp.x = ctrl + b.offset;
+        sp.n = b.ep.dim[a].n;
+        int64_t inner = 1;
+        for (int d = a + 1; d < b.dim; ++d) inner *= b.ep.dim[d].n;
+        sp.inner = inner;
+        sp.nlines = b.total / sp.n;
+        sp.lower = b.lower[a];
+        sp.upper = b.upper[a];
+        sp.diag = b.diag[a];
+        sp.bottom = b.bottom[a];
+        sp.right = b.right[a];
+        sp.lines_per_block = b.lines_per_block[a];
+        if (first_pass) {
+            sp.src = values + b.offset;
+            sp.outer_dims = a;
+            for (int d = 0; d < a; ++d) {
+                sp.outer_n[d] = b.ep.dim[d].n;
+                sp.outer_shift[d] = b.shift[d];
+            }
+            sp.shift_last = b.shift[a];
+        }
+        solve_tile<BW, CYC, double, double>(sp, w.first_line, tile_bytes, dst_base, src_base);
+        __syncthreads();
+    }
+}
+
+template <int BW>
+void launch_solve_group(bool cyc, int blocks, size_t lds, hipStream_t s, const BatchSpline* specs,
+                        const BatchSolveItem* items, int32_t n, const double* values, double* ctrl, int first) {
+    if (cyc)
+        hipLaunchKernelGGL((batch_solve_kernel<BW, true>), dim3(blocks), dim3(kTileThreads), lds, s, specs, items, n,
+                           values, ctrl, first);
+    else
+        hipLaunchKernelGGL((batch_solve_kernel<BW, false>), dim3(blocks), dim3(kTileThreads), lds, s, specs, items,
+                           n, values, ctrl, first);
+}
+
+hipError_t launch_solve_bw(int bw, bool cyc, int blocks, size_t lds, hipStream_t s, const BatchSpline* specs,
+                           const BatchSolveItem* items, int32_t n, const double* values, double* ctrl, int first) {
+    switch (bw) {
+        case 0: launch_solve_group<0>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 1: launch_solve_group<1>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 2: launch_solve_group<2>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 3: launch_solve_group<3>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 4: launch_solve_group<4>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 5: launch_solve_group<5>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        case 6: launch_solve_group<6>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
+        default: return hipErrorNotSupported;
+    }
+    return hipGetLastError();
+}
+
+// LDS budget of one solve workgroup (two per CU).
+constexpr int64_t kBatchTileBudget = 80 * 1024 - 2 * kMaxTileLines * 8;
+
+int64_t tile_fit(int64_t n, int cap) {
+    const int64_t row = (n | 1) * 8;
+    int lpb = kMaxTileLines;
+    while (lpb > 0 && (lpb > cap || lpb * row > kBatchTileBudget)) lpb /= 2;
+    return lpb;
+}
+
+struct SolveGroup {
+    int pass, bw;
+    bool cyc;
+    int64_t offset, count;
+    size_t lds;
+};
+
+struct EvalGroup {
+    int dim, order;
+    int64_t offset, count;
+};
+
+// Tunables (picked by the autotuner).
+struct Tuning {
+    int tile_cap = 64;           // lines per solve tile, at most
+    int solve_blocks_per_cu = 4;  // persistent solve workgroups per CU
+    int eval_threads = 256;       // points per eval item (= workgroup size)
+    int eval_blocks_per_cu = 8;   // persistent eval workgroups per CU
+};
+
+}  // namespace
+
+}  // namespace intp_amd
+
+using namespace intp_amd;
+
+struct intp_batch_s {
+    int device = 0;
+    int count = 0;
+    int num_cus = 256;
+    std::vector<DimSetup> setups;  // count * kMaxDim
+    std::vector<int> dims, orders;
+    std::vector<int64_t> offsets;  // count + 1
+    std::vector<unsigned char> image;
+    unsigned char* tables = nullptr;
+    std::vector<BatchSpline> host_specs;
+    BatchSpline* specs = nullptr;
+    double* ctrl = nullptr;
+    // solve work
+    std::vector<BatchSolveItem> solve_host;
+    BatchSolveItem* solve_items = nullptr;
+    int64_t solve_cap = 0;
+    std::vector<SolveGroup> solve_groups;
+    // eval work for the last point layout
+    std::vector<BatchEvalItem> eval_host;
+    BatchEvalItem* eval_items = nullptr;
+    int64_t eval_cap = 0;
+    std::vector<EvalGroup> eval_groups;
+    std::vector<int64_t> eval_layout;  // point offsets the eval items were made for
+    int eval_threads_used = 0;
+    // graph of one build + eval
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* graph_key[4] = {};
+    std::vector<int64_t> graph_layout;
+    Tuning tune;
+    bool tuned = false;
+    std::string tuning_report = "defaults (not tuned)";
+    UseTracker uses;
+
+    ~intp_batch_s() {
+        DeviceScope scope(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        release_after(device, tables, uses);
+        release_after(device, specs, uses);
+        release_after(device, ctrl, uses);
+        release_after(device, solve_items, uses);
+        release_after(device, eval_items, uses);
+    }
+};
+
+namespace {
+
+// (Re)builds the solve work items for the current tuning; uploads them on
+// `stream` (the host copy stays alive in the batch).
+int make_solve_work(intp_batch b, hipStream_t stream) {
+    // per spline tile sizes
+    for (int s = 0; s < b->count; ++s) {
+        BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
+        for (int d = 0; d < b->dims[static_cast<size_t>(s)]; ++d) {
+            const int64_t lpb = tile_fit(b->setups[static_cast<size_t>(s) * kMaxDim + d].n, b->tune.tile_cap);
+            if (lpb == 0) return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": line too long for a batch");
+            bs.lines_per_block[d] = static_cast<int32_t>(lpb);
+        }
+    }
+    b->solve_host.clear();
+    b->solve_groups.clear();
+    for (int pass = 0; pass < kMaxDim; ++pass) {
+        for (int bw = 0; bw <= 6; ++bw)
+            for (int cyc = 0; cyc <= 1; ++cyc) {
+                SolveGroup g{pass, bw, cyc != 0, static_cast<int64_t>(b->solve_host.size()), 0, 0};
+                for (int s = 0; s < b->count; ++s) {
+                    const int D = b->dims[static_cast<size_t>(s)];
+                    if (D <= pass) continue;
+                    const int a = D - 1 - pass;
+                    const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + a];
+                    if (ds.lu.p != bw || ds.lu.cyclic != (cyc != 0)) continue;
+                    const BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
+                    const int64_t lpb = bs.lines_per_block[a];
+                    const int64_t nlines = bs.total / ds.n;
+                    for (int64_t first = 0; first < nlines; first += lpb)
+                        b->solve_host.push_back({s, a, first});
+                    g.lds = std::max(g.lds, static_cast<size_t>(lpb * (ds.n | 1) * 8));
+                }
+                g.count = static_cast<int64_t>(b->solve_host.size()) - g.offset;
+                if (g.count > 0) b->solve_groups.push_back(g);
+            }
+    }
+    const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->solve_host.size()));
+    if (need > b->solve_cap) {
+        release_after(b->device, b->solve_items, b->uses);
+        b->solve_items = nullptr;
+        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->solve_items), need * sizeof(BatchSolveItem), stream),
+                "hipMallocAsync (batch work)");
+        b->solve_cap = need;
+    }
+    HIP_TRY(hipMemcpyAsync(b->specs, b->host_specs.data(), b->host_specs.size() * sizeof(BatchSpline),
+                           hipMemcpyHostToDevice, stream),
+            "hipMemcpyAsync");
+    if (!b->solve_host.empty())
+        HIP_TRY(hipMemcpyAsync(b->solve_items, b->solve_host.data(), b->solve_host.size() * sizeof(BatchSolveItem),
+                               hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return INTP_OK;
+}
+
+// Eval work items for a point layout (point_offsets[count + 1]).
+int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t stream) {
+    const std::vector<int64_t> layout(point_offsets, point_offsets + b->count + 1);
+    if (layout == b->eval_layout && b->eval_threads_used == b->tune.eval_threads) return INTP_OK;
+    for (int s = 0; s < b->count; ++s)
+        if (layout[static_cast<size_t>(s) + 1] < layout[static_cast<size_t>(s)])
+            return fail(INTP_E_INVALID, "point offsets must be non-decreasing");
+    b->eval_host.clear();
+    b->eval_groups.clear();
+    const int64_t per = b->tune.eval_threads;
+    for (int D = 1; D <= kMaxDim; ++D)
+        for (int K = 0; K <= kMaxOrder; ++K) {
+            EvalGroup g{D, K, static_cast<int64_t>(b->eval_host.size()), 0};
+            for (int s = 0; s < b->count; ++s) {
+                if (b->dims[static_cast<size_t>(s)] != D || b->orders[static_cast<size_t>(s)] != K) continue;
+                const int64_t p0 = layout[static_cast<size_t>(s)], p1 = layout[static_cast<size_t>(s) + 1];
+                for (int64_t p = p0; p < p1; p += per)
+                    b->eval_host.push_back({s, static_cast<int32_t>(std::min(per, p1 - p)), p});
+            }
+            g.count = static_cast<int64_t>(b->eval_host.size()) - g.offset;
+            if (g.count > 0) b->eval_groups.push_back(g);
+        }
+    const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->eval_host.size()));
+    if (need > b->eval_cap) {
+        release_after(b->device, b->eval_items, b->uses);
+        b->eval_items = nullptr;
+        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->eval_items), need * sizeof(BatchEvalItem), stream),
+                "hipMallocAsync (batch work)");
+        b->eval_cap = need;
+    }
+    if (!b->eval_host.empty())
+        HIP_TRY(hipMemcpyAsync(b->eval_items, b->eval_host.data(), b->eval_host.size() * sizeof(BatchEvalItem),
+                               hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    b->eval_layout = layout;
+    b->eval_threads_used = b->tune.eval_threads;
+    return INTP_OK;
+}
+
+int launch_build(intp_batch b, const double* values, hipStream_t stream) {
+    for (const SolveGroup& g : b->solve_groups) {
+        const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
+        const hipError_t e = launch_solve_bw(g.bw, g.cyc, blocks, g.lds, stream, b->specs, b->solve_items + g.offset,
+                                             static_cast<int32_t>(g.count), values, b->ctrl, g.pass == 0 ? 1 : 0);
+        if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
+        HIP_TRY(e, "batch solve launch");
+    }
+    return INTP_OK;
+}
+
+int launch_eval_work(intp_batch b, const double* coords, double* out, hipStream_t stream) {
+    for (const EvalGroup& g : b->eval_groups) {
+        BatchEvalArgs a{b->specs, b->eval_items + g.offset, static_cast<int32_t>(g.count), coords, out};
+        const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.eval_blocks_per_cu));
+        hipError_t e = hipErrorNotSupported;
+        if (g.dim == 1) e = batch_eval_dispatch_d1(g.order, a, blocks, b->tune.eval_threads, stream);
+        if (g.dim == 2) e = batch_eval_dispatch_d2(g.order, a, blocks, b->tune.eval_threads, stream);
+        if (g.dim == 3) e = batch_eval_dispatch_d3(g.order, a, blocks, b->tune.eval_threads, stream);
+        if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "no batched kernel for this dimension/order");
+        HIP_TRY(e, "batch eval launch");
+    }
+    return INTP_OK;
+}
+
+void drop_graph(intp_batch b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Times one build + eval with the current tuning (median of 3, ms).
+float time_run(intp_batch b, const double* values, const double* coords, double* out, hipStream_t stream,
+               int* rc) {
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float t[3] = {0, 0, 0};
+    *rc = launch_build(b, values, stream);
+    if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);  // warm
+    for (int r = 0; r < 3 && *rc == INTP_OK; ++r) {
+        (void)hipEventRecord(e0, stream);
+        *rc = launch_build(b, values, stream);
+        if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);
+        (void)hipEventRecord(e1, stream);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&t[r], e0, e1);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::sort(t, t + 3);
+    return t[1];
+}
+
+// Coordinate descent over the tunables, each candidate timed on the
+// batch's own build + eval.
+int autotune(intp_batch b, const double* values, const double* coords, const int64_t* point_offsets, double* out,
+             hipStream_t stream) {
+    std::string report;
+    auto try_values = [&](const char* name, int Tuning::*field, std::initializer_list<int> cands) -> int {
+        int best = b->tune.*field;
+        float best_ms = 1e30f;
+        std::string line = std::string(name) + ":";
+        for (int v : cands) {
+            b->tune.*field = v;
+            int rc = make_solve_work(b, stream);
+            if (rc == INTP_OK) rc = make_eval_work(b, point_offsets, stream);
+            if (rc != INTP_OK) return rc;
+            const float ms = time_run(b, values, coords, out, stream, &rc);
+            if (rc != INTP_OK) return rc;
+            line += " " + std::to_string(v) + "=" + std::to_string(ms).substr(0, 6) + "ms";
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        b->tune.*field = best;
+        report += line + " -> " + std::to_string(best) + "; ";
+        return INTP_OK;
+    };
+    int rc;
+    if ((rc = try_values("eval_threads", &Tuning::eval_threads, {64, 128, 256})) != INTP_OK) return rc;
+    if ((rc = try_values("eval_blocks_per_cu", &Tuning::eval_blocks_per_cu, {2, 4, 8, 16})) != INTP_OK) return rc;
+    if ((rc = try_values("tile_cap", &Tuning::tile_cap, {8, 16, 32, 64})) != INTP_OK) return rc;
+    if ((rc = try_values("solve_blocks_per_cu", &Tuning::solve_blocks_per_cu, {1, 2, 4, 8})) != INTP_OK) return rc;
+    if ((rc = make_solve_work(b, stream)) != INTP_OK) return rc;
+    if ((rc = make_eval_work(b, point_offsets, stream)) != INTP_OK) return rc;
+    b->tuned = true;
+    b->tuning_report = report;
+    drop_graph(b);
+    return INTP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs, intp_batch* out) {
+    if (!ctx_ || !descs || !out || count < 1) return fail(INTP_E_INVALID, "null argument or empty batch");
+    const int device = ctx_->device;
+    std::unique_ptr<intp_batch_s> b(new intp_batch_s());
+    b->device = device;
+    b->count = count;
+    b->setups.resize(static_cast<size_t>(count) * kMaxDim);
+    b->dims.resize(static_cast<size_t>(count));
+    b->orders.resize(static_cast<size_t>(count));
+    for (int s = 0; s < count; ++s) {
+        const intp_grid_desc& g = descs[s];
+        if (g.dim < 1 || g.dim > kMaxDim || g.order < 0 || g.order > kMaxOrder)
+            return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": no kernels for this dimension/order");
+        b->dims[static_cast<size_t>(s)] = g.dim;
+        b->orders[static_cast<size_t>(s)] = g.order;
+    }
+    // host setup (knots, collocation LU) of every dimension, on a thread pool
+    std::atomic<int> next{0};
+    std::vector<std::string> errors(static_cast<size_t>(count));
+    auto worker = [&]() {
+        for (int s = next.fetch_add(1); s < count; s = next.fetch_add(1)) {
+            const intp_grid_desc& g = descs[s];
+            for (int d = 0; d < g.dim; ++d) {
+                const std::string err =
+                    setup_dimension(g.order, g.periodic[d] != 0, g.n[d], g.xmin[d], g.xmax[d], g.coords[d],
+                                    b->setups[static_cast<size_t>(s) * kMaxDim + d]);
+                if (!err.empty()) {
+                    errors[static_cast<size_t>(s)] = err + " (dimension " + std::to_string(d) + ")";
+                    break;
+                }
+            }
+        }
+    };
+    const int nthreads = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    for (int s = 0; s < count; ++s)
+        if (!errors[static_cast<size_t>(s)].empty())
+            return fail(INTP_E_INVALID, "spline " + std::to_string(s) + ": " + errors[static_cast<size_t>(s)]);
+
+    DeviceScope scope(device);
+    if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
+    DeviceRuntime* rt = device_runtime(device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    hipStream_t stream = rt->setup;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        b->num_cus = cus;
+
+    // offsets and the packed table image
+    b->offsets.assign(static_cast<size_t>(count) + 1, 0);
+    struct Item {
+        const void* src;
+        size_t bytes;
+        size_t at;
+    };
+    std::vector<Item> items;
+    size_t image_bytes = 0;
+    auto place = [&](const auto& v) -> size_t {
+        const size_t at = image_bytes;
+        const size_t bytes = v.size() * sizeof(v[0]);
+        items.push_back({v.data(), bytes, at});
+        image_bytes += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    struct Where {
+        size_t knots, recip, poly, lower, upper, diag, bottom, right;
+    };
+    std::vector<Where> where(static_cast<size_t>(count) * kMaxDim);
+    for (int s = 0; s < count; ++s) {
+        int64_t total = 1;
+        for (int d = 0; d < b->dims[static_cast<size_t>(s)]; ++d) {
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + d];
+            total *= ds.n;
+            Where& w = where[static_cast<size_t>(s) * kMaxDim + d];
+            w.knots = place(ds.knots);
+            w.recip = place(ds.recip);
+            w.poly = place(ds.poly);
+            w.lower = place(ds.lu.lower);
+            w.upper = place(ds.lu.upper);
+            w.diag = place(ds.lu.diag);
+            w.bottom = place(ds.lu.bottom);
+            w.right = place(ds.lu.right);
+        }
+        b->offsets[static_cast<size_t>(s) + 1] = b->offsets[static_cast<size_t>(s)] + total;
+    }
+    const int64_t total = b->offsets.back();
+    if (total >= (int64_t{1} << 32)) return fail(INTP_E_UNSUPPORTED, "a batch holds fewer than 2^32 control points");
+    b->image.assign(std::max<size_t>(image_bytes, 256), 0);
+    for (const Item& it : items)
+        if (it.bytes) std::memcpy(b->image.data() + it.at, it.src, it.bytes);
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->tables), b->image.size(), stream), "hipMallocAsync (batch)");
+    HIP_TRY(hipMemcpyAsync(b->tables, b->image.data(), b->image.size(), hipMemcpyHostToDevice, stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->specs), static_cast<size_t>(count) * sizeof(BatchSpline), stream),
+            "hipMallocAsync (batch)");
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->ctrl), static_cast<size_t>(std::max<int64_t>(total, 1)) * 8,
+                           stream),
+            "hipMallocAsync (batch control points)");
+    HIP_TRY(b->uses.record(stream), "hipEventRecord");
+
+    // device descriptors
+    b->host_specs.assign(static_cast<size_t>(count), BatchSpline{});
+    auto dptr = [&](size_t at, size_t bytes) -> const double* {
+        return bytes ? reinterpret_cast<const double*>(b->tables + at) : nullptr;
+    };
+    for (int s = 0; s < count; ++s) {
+        BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
+        const int D = b->dims[static_cast<size_t>(s)], K = b->orders[static_cast<size_t>(s)];
+        bs.dim = D;
+        bs.order = K;
+        bs.offset = b->offsets[static_cast<size_t>(s)];
+        bs.total = b->offsets[static_cast<size_t>(s) + 1] - bs.offset;
+        EvalParams& ep = bs.ep;
+        for (int d = 0; d < D; ++d) {
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + d];
+            const Where& w = where[static_cast<size_t>(s) * kMaxDim + d];
+            int64_t stride = 1;
+            for (int e = d + 1; e < D; ++e) stride *= b->setups[static_cast<size_t>(s) * kMaxDim + e].n;
+            ep.dim[d] = make_dim_params(ds, K, stride, dptr(w.knots, ds.knots.size()), dptr(w.recip, ds.recip.size()),
+                                        dptr(w.poly, ds.poly.size()));
+            bs.lower[d] = dptr(w.lower, ds.lu.lower.size());
+            bs.upper[d] = dptr(w.upper, ds.lu.upper.size());
+            bs.diag[d] = dptr(w.diag, ds.lu.diag.size());
+            bs.bottom[d] = dptr(w.bottom, ds.lu.bottom.size());
+            bs.right[d] = dptr(w.right, ds.lu.right.size());
+            bs.shift[d] = ds.periodic ? (K / 2) % ds.n : 0;
+        }
+        ep.ctrl = b->ctrl + bs.offset;
+        ep.cs_dim = 1;
+        ep.cs_pt = 3;
+        ep.os_out = 0;
+        ep.os_pt = 1;
+        for (int i = 0; i <= kMaxDim; ++i) ep.slot[i] = -1;
+        ep.out_index = 0;
+    }
+    const int rc = make_solve_work(b.get(), stream);
+    if (rc != INTP_OK) return rc;
+    *out = b.release();
+    return INTP_OK;
+}
+
+int intp_batch_destroy(intp_batch b) {
+    delete b;
+    return INTP_OK;
+}
+
+int intp_batch_info(intp_batch b, int64_t* values_total, int64_t* value_offsets) {
+    if (!b) return fail(INTP_E_INVALID, "null batch");
+    if (values_total) *values_total = b->offsets.back();
+    if (value_offsets) std::memcpy(value_offsets, b->offsets.data(), b->offsets.size() * sizeof(int64_t));
+    return INTP_OK;
+}
+
+int intp_batch_build(intp_batch b, const double* values, void* stream_) {
+    if (!b || !values) return fail(INTP_E_INVALID, "null argument");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    DeviceScope scope(b->device);
+    const int rc = launch_build(b, values, stream);
+    const hipError_t e = b->uses.record(stream);
+    if (rc != INTP_OK) return rc;
+    HIP_TRY(e, "hipEventRecord");
+    return INTP_OK;
+}
+
+int intp_batch_eval(intp_batch b, const double* coords, const int64_t* point_offsets, double* out, void* stream_) {
+    if (!b || !coords || !point_offsets || !out) return fail(INTP_E_INVALID, "null argument");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    DeviceScope scope(b->device);
+    int rc = make_eval_work(b, point_offsets, stream);
+    if (rc != INTP_OK) return rc;
+    rc = launch_eval_work(b, coords, out, stream);
+    const hipError_t e = b->uses.record(stream);
+    if (rc != INTP_OK) return rc;
+    HIP_TRY(e, "hipEventRecord");
+    return INTP_OK;
+}
+
+int intp_batch_run(intp_batch b, const double* values, const double* coords, const int64_t* point_offsets,
+                   double* out, uint32_t flags, void* stream_) {
+    if (!b || !values || !coords || !point_offsets || !out) return fail(INTP_E_INVALID, "null argument");
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    DeviceScope scope(b->device);
+    int rc = INTP_OK;
+    if ((flags & INTP_BATCH_AUTOTUNE) && !b->tuned)
+        if ((rc = autotune(b, values, coords, point_offsets, out, stream)) != INTP_OK) return rc;
+    if ((rc = make_eval_work(b, point_offsets, stream)) != INTP_OK) return rc;
+    if ((flags & INTP_BATCH_GRAPH) && stream != nullptr) {
+        const void* key[4] = {values, coords, out, stream};
+        const bool same = b->exec && std::equal(key, key + 4, b->graph_key) && b->graph_layout == b->eval_layout;
+        if (!same) {
+            drop_graph(b);
+            HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            rc = launch_build(b, values, stream);
+            if (rc == INTP_OK) rc = launch_eval_work(b, coords, out, stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(stream, &g);
+            if (rc != INTP_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            HIP_TRY(ec, "hipStreamEndCapture");
+            b->graph = g;
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            std::copy(key, key + 4, b->graph_key);
+            b->graph_layout = b->eval_layout;
+        }
+        HIP_TRY(hipGraphLaunch(b->exec, stream), "hipGraphLaunch");
+    } else {
+        rc = launch_build(b, values, stream);
+        if (rc == INTP_OK) rc = launch_eval_work(b, coords, out, stream);
+        if (rc != INTP_OK) return rc;
+    }
+    HIP_TRY(b->uses.record(stream), "hipEventRecord");
+    return INTP_OK;
+}
+
+int intp_batch_tuning(intp_batch b, char* buf, int64_t cap) {
+    if (!b || !buf || cap < 1) return fail(INTP_E_INVALID, "null argument");
+    const std::string s = "tile_cap=" + std::to_string(b->tune.tile_cap) +
+                          " solve_blocks_per_cu=" + std::to_string(b->tune.solve_blocks_per_cu) +
+                          " eval_threads=" + std::to_string(b->tune.eval_threads) +
+                          " eval_blocks_per_cu=" + std::to_string(b->tune.eval_blocks_per_cu) + " | " +
+                          b->tuning_report;
+    const size_t n = std::min<size_t>(s.size(), static_cast<size_t>(cap - 1));
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+    return INTP_OK;
+}
+
+int intp_batch_download_ctrl(intp_batch b, int32_t spline, double* host, int64_t count) {
+    if (!b || !host) return fail(INTP_E_INVALID, "null argument");
+    if (spline < 0 || spline >= b->count) return fail(INTP_E_INVALID, "no such spline");
+    const int64_t off = b->offsets[static_cast<size_t>(spline)];
+    if (count != b->offsets[static_cast<size_t>(spline) + 1] - off)
+        return fail(INTP_E_INVALID, "count must equal the spline's control points");
+    DeviceScope scope(b->device);
+    DeviceRuntime* rt = device_runtime(b->device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    b->uses.order_before(rt->setup);
+    HIP_TRY(hipMemcpyAsync(host, b->ctrl + off, static_cast<size_t>(count) * 8, hipMemcpyDeviceToHost, rt->setup),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
+    return INTP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// batch_impl.hpp -- many independent FP64 splines built and evaluated by
+// the same launches (the batched form): device-side descriptors, work items
+// and the batched evaluation kernel.  Each spline keeps the single-spline
+// arithmetic: its lines are solved by solve_tile (solve_impl.hpp) and its
+// points evaluated by uniform_point / general_point (eval_impl.hpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace intp_amd {
+
+// One spline of a batch, in device memory.
+struct BatchSpline {
+    EvalParams ep;  // dims, control points (ctrl), value-only output layout
+    const double* lower[kMaxDim];
+    const double* upper[kMaxDim];
+    const double* diag[kMaxDim];
+    const double* bottom[kMaxDim];
+    const double* right[kMaxDim];
+    int64_t offset;  // first value / control point of the spline in the batch arrays
+    int64_t total;   // its control points
+    int64_t shift[kMaxDim];
+    int32_t dim, order;
+    int32_t lines_per_block[kMaxDim];  // LDS tile of the solve along each axis
+};
+
+// A tile of lines of one spline along one axis.
+struct BatchSolveItem {
+    int32_t spline;
+    int32_t axis;
+    int64_t first_line;
+};
+
+// A run of points of one spline: points [first, first + count) of the
+// batch's point arrays.
+struct BatchEvalItem {
+    int32_t spline;
+    int32_t count;
+    int64_t first;
+};
+
+struct BatchEvalArgs {
+    const BatchSpline* specs;
+    const BatchEvalItem* items;
+    int32_t nitems;
+    const double* coords;  // [points][3] (x, y, z; unused dimensions ignored)
+    double* out;           // [points]
+};
+
+hipError_t batch_eval_dispatch_d1(int order, const BatchEvalArgs& a, int blocks, int threads, hipStream_t s);
+hipError_t batch_eval_dispatch_d2(int order, const BatchEvalArgs& a, int blocks, int threads, hipStream_t s);
+hipError_t batch_eval_dispatch_d3(int order, const BatchEvalArgs& a, int blocks, int threads, hipStream_t s);
+
+#ifdef INTP_BATCH_EVAL_KERNELS
+
+namespace {
+
+// Persistent: each workgroup takes items blockIdx.x, + gridDim.x, ...; the
+// item's spline parameters are copied into LDS, then one thread per point.
+template <int D, int K>
+__global__ __launch_bounds__(256) void batch_eval_kernel(BatchEvalArgs a) {
+    constexpr int kCoef = (K + 1) * (K + 1);
+    __shared__ EvalParams sp;
+    __shared__ double s_poly[D][(1 + 2 * K) * kCoef];
+    for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+        const BatchEvalItem w = a.items[it];
+        const BatchSpline* __restrict__ b = a.specs + w.spline;
+        {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(&b->ep);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(&sp);
+            for (int i = threadIdx.x; i < static_cast<int>(sizeof(EvalParams) / 4); i += blockDim.x) dst[i] = src[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            sp.coords = a.coords;
+            sp.out = a.out;
+        }
+        const bool uniform = sp.dim[0].uniform && (D < 2 || sp.dim[1].uniform) && (D < 3 || sp.dim[2].uniform) &&
+                             sp.dim[0].poly != nullptr;
+        if (uniform) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int n = sp.dim[d].nclass * kCoef;
+                for (int i = threadIdx.x; i < n; i += blockDim.x) s_poly[d][i] = sp.dim[d].poly[i];
+            }
+        }
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < w.count) {
+            const int64_t pt = w.first + threadIdx.x;
+            const bool lastc = !sp.dim[D - 1].periodic;
+            if (uniform) {
+                if (lastc)
+                    uniform_point<D, K, false, true, double>(sp, s_poly, pt);
+                else
+                    uniform_point<D, K, false, false, double>(sp, s_poly, pt);
+            } else {
+                if (lastc)
+                    general_point<D, K, false, true, double>(sp, pt);
+                else
+                    general_point<D, K, false, false, double>(sp, pt);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int D>
+hipError_t batch_eval_dispatch(int order, const BatchEvalArgs& a, int blocks, int threads, hipStream_t s) {
+    if (a.nitems == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(blocks)), block(static_cast<unsigned>(threads));
+    switch (order) {
+        case 0: hipLaunchKernelGGL((batch_eval_kernel<D, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((batch_eval_kernel<D, 1>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((batch_eval_kernel<D, 2>), grid, block, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((batch_eval_kernel<D, 3>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((batch_eval_kernel<D, 4>), grid, block, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((batch_eval_kernel<D, 5>), grid, block, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((batch_eval_kernel<D, 6>), grid, block, 0, s, a); break;
+        case 7: hipLaunchKernelGGL((batch_eval_kernel<D, 7>), grid, block, 0, s, a); break;
+        default: return hipErrorNotSupported;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+#endif  // INTP_BATCH_EVAL_KERNELS
+
+}  // namespace intp_amd

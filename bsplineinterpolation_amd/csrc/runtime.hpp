@@ -11,6 +11,11 @@
 
 #include "../../include/intp_amd.h"
 
+// A device context (intp_ctx_create).
+struct intp_ctx_s {
+    int device;
+};
+
 namespace intp_amd {
 
 inline thread_local std::string g_error;

@@ -16,6 +16,7 @@ import numpy as np
 SEED = 0x20261202
 STREAM_NOISE = 0
 STREAM_QUERY = 1
+STREAM_SPEC = 2
 
 _GOLD = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -143,3 +144,45 @@ def workload_1d_periodic_sextic(seed: int = SEED, n: int = 8192, nq: int = 1 << 
     f = np.sin(x) + 0.3 * np.cos(5 * x) + normal(seed, STREAM_NOISE, n, 1e-3)
     q = 2 * math.pi * uniform(seed, STREAM_QUERY, nq).reshape(1, nq)
     return dict(order=6, f=f, periodic=[True], ranges=[(0.0, 2 * math.pi)], queries=q, orders=[[0]])
+
+
+def workload_batch_mixed(seed: int = SEED, count: int = 4096, queries_per_spline: int = 256):
+    """The batched workload: ``count`` FP64 splines, half 1D with lengths
+    U{16..4096}, 3/8 2D with sides U{16..256}, 1/8 3D with sides U{8..64};
+    orders U{1..5}; each dimension periodic with probability 1/4; ranges
+    [0, 1].  Values: a sum of 3 sinusoids with seeded amplitudes, frequencies
+    and phases plus N(0, 1e-2^2); ``queries_per_spline`` uniform points each
+    (value).  Returns specs, concatenated values, points [npts, 3] and
+    point offsets."""
+    n1, n2 = count // 2, (count * 3) // 8
+    u = uniform(seed, STREAM_SPEC, 16 * count).reshape(count, 16)
+
+    def pick(v, lo, hi):
+        return lo + int(v * (hi - lo + 1)) if v < 1.0 else hi
+
+    specs, vals, pts, offs = [], [], [], [0]
+    for i in range(count):
+        r = u[i]
+        D = 1 if i < n1 else (2 if i < n1 + n2 else 3)
+        lo_n, hi_n = ((16, 4096), (16, 256), (8, 64))[D - 1]
+        shape = tuple(pick(r[d], lo_n, hi_n) for d in range(D))
+        order = pick(r[3], 1, 5)
+        periodic = [bool(r[4 + d] < 0.25) for d in range(D)]
+        # a periodic dimension needs n >= 2*floor(k/2) + 2
+        shape = tuple(max(s, 2 * (order // 2) + 2) if p else max(s, order + 1) for s, p in zip(shape, periodic))
+        specs.append(dict(order=order, shape=shape, periodic=periodic, ranges=[(0.0, 1.0)] * D))
+        axes = [np.arange(n) / n if p else np.linspace(0.0, 1.0, n) for n, p in zip(shape, periodic)]
+        grid = np.meshgrid(*axes, indexing="ij")
+        f = np.zeros(shape)
+        for m in range(3):
+            amp = 0.5 + r[7 + m]
+            arg = np.zeros(shape)
+            for d in range(D):
+                arg = arg + (1.0 + 5.0 * r[10 + (m + d) % 3]) * grid[d]
+            f = f + amp * np.sin(2 * math.pi * arg + 6.28 * r[13 + m])
+        vals.append(f.ravel())
+        offs.append(offs[-1] + queries_per_spline)
+    pts = [uniform(seed, STREAM_QUERY, 3 * offs[-1]).reshape(offs[-1], 3)]
+    values = np.concatenate(vals)
+    values = values + normal(seed, STREAM_NOISE, values.size, 1e-2)
+    return dict(specs=specs, values=values, points=np.concatenate(pts), point_offsets=np.asarray(offs, np.int64))

@@ -180,6 +180,40 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
               int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
               uint32_t flags, void* stream);
 
+/* ---- batches: many independent splines built and evaluated together -- */
+/* No single reference counterpart: a batch is N independent
+ * InterpolationFunction<double, D_s, O_s> objects (Interpolation.hpp:92-107)
+ * of mixed dimension, order, size and periodicity, each built and evaluated
+ * with the single-spline arithmetic (its control points equal
+ * intp_build's), but by a fixed set of persistent launches over all of
+ * them.  Layouts:
+ *   values / control points: the splines' C-order meshes concatenated in
+ *     spline order (intp_batch_info gives the offsets);
+ *   points: coords[i*3 + d] (x, y, z; unused dimensions ignored), spline s
+ *     owning points point_offsets[s] .. point_offsets[s+1]-1 (host array of
+ *     count + 1); out[i] its value.                                        */
+typedef struct intp_batch_s* intp_batch;
+#define INTP_BATCH_GRAPH 1u    /* capture build + eval into one hipGraph and
+                                  replay it while the buffers, the stream and
+                                  the point layout stay the same (stream must
+                                  not be the null stream)                  */
+#define INTP_BATCH_AUTOTUNE 2u /* on the first run: time candidate tile
+                                  shapes (solve lines per tile, persistent
+                                  workgroups per CU, points per eval
+                                  workgroup) on this batch, keep the fastest */
+int intp_batch_create(intp_ctx ctx, int32_t count, const intp_grid_desc* descs, intp_batch* out);
+int intp_batch_destroy(intp_batch b);
+/* Total values (= control points) and the per-spline offsets (count + 1). */
+int intp_batch_info(intp_batch b, int64_t* values_total, int64_t* value_offsets);
+int intp_batch_build(intp_batch b, const double* values, void* stream);
+int intp_batch_eval(intp_batch b, const double* coords, const int64_t* point_offsets, double* out, void* stream);
+/* Build then evaluate (values, coords, out: device). */
+int intp_batch_run(intp_batch b, const double* values, const double* coords, const int64_t* point_offsets,
+                   double* out, uint32_t flags, void* stream);
+/* The tile shapes in use and the autotuner's timings, as text. */
+int intp_batch_tuning(intp_batch b, char* buf, int64_t capacity);
+int intp_batch_download_ctrl(intp_batch b, int32_t spline, double* host, int64_t count);
+
 /* Durations (ms) of the calls made with INTP_EVAL_TIMED on this thread
  * since the last query, oldest first: group_ms[i] covers the grouping
  * pass (0 when none ran), kernel_ms[i] the evaluation kernels alone, each

@@ -1,0 +1,105 @@
+"""Batched form (intp_batch_*): every spline of a mixed batch must come out
+exactly as the single-spline path computes it (control points and values
+bit for bit), graph replays must follow new buffer contents, and the
+autotuner must not change results."""
+import numpy as np
+import pytest
+
+from cases import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import bsplineinterpolation_amd as amd
+    return amd
+
+
+@pytest.fixture(scope="module")
+def work():
+    from bsplineinterpolation_amd import datagen
+    return datagen.workload_batch_mixed(seed=7, count=48, queries_per_spline=300)
+
+
+def _setup(amd, work):
+    import torch
+    b = amd.Batch(work["specs"])
+    assert b.total == work["values"].size
+    vals = torch.from_numpy(work["values"]).cuda()
+    pts = torch.from_numpy(np.ascontiguousarray(work["points"])).cuda()
+    return b, vals, pts
+
+
+def test_gpu_batch_equals_single_splines(amd, work, orc):
+    import torch
+    b, vals, pts = _setup(amd, work)
+    out = b.run(vals, pts, work["point_offsets"])
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    po = work["point_offsets"]
+    for s, sp in enumerate(work["specs"]):
+        f = work["values"][b.offsets[s]:b.offsets[s + 1]].reshape(sp["shape"])
+        g = amd.InterpolationFunction(sp["order"], f, periodic=sp["periodic"], ranges=sp["ranges"])
+        assert np.array_equal(b.control_points(s), g.control_points().ravel()), s
+        D = len(sp["shape"])
+        q = work["points"][po[s]:po[s + 1], :D]
+        assert np.array_equal(got[po[s]:po[s + 1]], g.eval(q)[0]), s
+    # and the oracle for a few splines of each dimension
+    for s in (0, 30, 45):
+        sp = work["specs"][s]
+        f = work["values"][b.offsets[s]:b.offsets[s + 1]].reshape(sp["shape"])
+        o = orc.OracleSpline(sp["order"], f, periodic=sp["periodic"], ranges=sp["ranges"])
+        D = len(sp["shape"])
+        want = o.eval(np.ascontiguousarray(work["points"][po[s]:po[s + 1], :D]))[0]
+        assert rel_err(got[po[s]:po[s + 1]], want) < 1e-12
+
+
+def test_gpu_batch_graph_replay_and_autotune(amd, work):
+    import torch
+    b, vals, pts = _setup(amd, work)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        plain = b.run(vals, pts, work["point_offsets"]).clone()
+        out = torch.empty_like(plain)
+        b.run(vals, pts, work["point_offsets"], out=out, graph=True)
+        first = out.clone()
+        b.run(vals, pts, work["point_offsets"], out=out, graph=True)  # replay
+        second = out.clone()
+        vals.mul_(2.0)  # exact scaling: a linear spline of 2 f is exactly 2 s(f)
+        b.run(vals, pts, work["point_offsets"], out=out, graph=True)  # replay sees new values
+        doubled = out.clone()
+        vals.mul_(0.5)
+        tuned = b.run(vals, pts, work["point_offsets"], autotune=True).clone()
+    stream.synchronize()
+    assert torch.equal(first, plain) and torch.equal(second, plain)
+    assert torch.equal(doubled, 2.0 * plain)
+    assert torch.equal(tuned, plain)
+    rep = b.tuning()
+    assert "tile_cap=" in rep and "->" in rep, rep
+    print(rep)
+
+
+def test_gpu_batch_separate_build_and_eval(amd, work):
+    import torch
+    b, vals, pts = _setup(amd, work)
+    plain = b.run(vals, pts, work["point_offsets"]).clone()
+    b.build(vals)
+    again = b.eval(pts, work["point_offsets"])
+    # a different point layout: every other spline gets no points
+    po = work["point_offsets"]
+    counts = np.diff(po).copy()
+    counts[1::2] = 0
+    po2 = np.concatenate([[0], np.cumsum(counts)])
+    sel = np.concatenate([np.arange(po[s], po[s + 1]) for s in range(0, len(counts), 2)])
+    sub = b.eval(pts[torch.from_numpy(sel).cuda()].contiguous(), po2)
+    torch.cuda.synchronize()
+    assert torch.equal(again, plain)
+    assert torch.equal(sub, plain[torch.from_numpy(sel).cuda()])
+
+
+def test_gpu_batch_rejects_bad_specs(amd):
+    with pytest.raises(amd.IntpError):
+        amd.Batch([dict(order=3, shape=(10,), ranges=[(0, 1)]), dict(order=9, shape=(10,), ranges=[(0, 1)])])
+    with pytest.raises(amd.IntpError):
+        amd.Batch([dict(order=4, shape=(4,), periodic=[True], ranges=[(0, 1)])])
