@@ -147,6 +147,55 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
             "periodic_1d_order6_n8192": {"error_vs_f64": err1, "note": "periodic order >= 5: FP64 factors"}}
 
 
+def measure_batched(amd, torch, steps: int):
+    """The batched form: 4096 mixed 1D/2D/3D FP64 splines (orders 1..5,
+    ~25% periodic dimensions), built and evaluated at 256 points each per
+    run; runs replay one captured hipGraph after an autotuned first run.
+    Throughput counts control points solved and points evaluated."""
+    from bsplineinterpolation_amd import datagen
+    w = datagen.workload_batch_mixed()
+    dev = torch.device("cuda:0")
+    vals = torch.from_numpy(w["values"]).to(dev)
+    pts = torch.from_numpy(np.ascontiguousarray(w["points"])).to(dev)
+    po = w["point_offsets"]
+    out = torch.empty(int(po[-1]), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = amd.Batch(w["specs"])
+    create_ms = (time.perf_counter() - t0) * 1e3
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        t0 = time.perf_counter()
+        b.run(vals, pts, po, out=out, autotune=True)
+        stream.synchronize()
+        tune_ms = (time.perf_counter() - t0) * 1e3
+        b.run(vals, pts, po, out=out, graph=True)  # capture
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.run(vals, pts, po, out=out, graph=True)
+        e1.record(stream)
+        stream.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    run_ms = e0.elapsed_time(e1) / steps
+    N = int(b.total)
+    Q = int(po[-1])
+    dims = [len(sp["shape"]) for sp in w["specs"]]
+    sizes = np.diff(b.offsets)
+    b_solve = int(sum(2 * D * n * 8 for D, n in zip(dims, sizes)))
+    b_eval = int(sum((D * 8 + 8) * q for D, q in zip(dims, np.diff(po)))) + N * 8
+    return {"workload": "4096 FP64 splines: 2048 1D (n 16..4096), 1536 2D (16..256 per side), 512 3D (8..64); "
+                        "orders 1..5, ~25% periodic dims; 256 value queries each",
+            "control_points": N, "points": Q, "run_ms": run_ms, "wall_ms_per_run": wall_ms,
+            "splines_per_s": len(dims) / (run_ms * 1e-3), "control_points_per_s": N / (run_ms * 1e-3),
+            "points_per_s": Q / (run_ms * 1e-3), "algorithmic_bytes": b_solve + b_eval,
+            "gbs": (b_solve + b_eval) / (run_ms * 1e-3) / 1e9, "create_ms": create_ms, "autotune_ms": tune_ms,
+            "tuning": b.tuning().split(" | ")[0], "graph": True,
+            "parity": "control points and values bit-identical to the single-spline path (tests/test_gpu_batch.py)"}
+
+
 def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
     """Reference CPU path, bounded sample.  Returns (dict, error stats)."""
     import oracle
@@ -189,6 +238,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the refined-FP32 form measurement")
     ap.add_argument("--only-forms", action="store_true",
                     help="measure only the refined-FP32 form (profiling runs); prints its object")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="measure only the batched form (4096 mixed splines); prints its object")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="copy 256 MiB once before timing (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -202,8 +253,13 @@ def main():
     from bsplineinterpolation_amd import datagen
 
     n, nq = args.n, args.queries
+    if args.only_batched:
+        print(json.dumps(measure_batched(amd, torch, args.steps)))
+        return
     if args.only_forms:
-        print(json.dumps(measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)))
+        forms = measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)
+        forms["batched"] = measure_batched(amd, torch, args.steps)
+        print(json.dumps(forms))
         return
     print("bench: generating inputs (%d^3 mesh, %d queries)" % (n, nq), file=sys.stderr, flush=True)
     w = datagen.workload_3d_256_cubic(n=n, nq=nq)
@@ -310,6 +366,8 @@ def main():
         del keep
         keep = None
         result["forms"] = measure_refined_form(amd, torch, max(3, args.steps // 2), 2, n, nq)
+        print("bench: batched form (4096 mixed splines)", file=sys.stderr, flush=True)
+        result["forms"]["batched"] = measure_batched(amd, torch, max(5, args.steps))
 
     if not args.no_cpu:
         m = min(args.cpu_sample, nq)

@@ -290,6 +290,12 @@ int solve_axes(const GridData& g, bool refined, const double* values, void* x, v
         sp.bottom = f32 ? static_cast<const void*>(g.bottom32[a]) : g.bottom[a];
         sp.right = f32 ? static_cast<const void*>(g.right32[a]) : g.right[a];
         sp.lines_per_block = solve_tile_lines(s.n, compute_bytes(types));
+        // Strided axes with a wavefront or more of adjacent lines: lane l's
+        // line sits next to lane l+1's, so substituting in place reads and
+        // writes coalesced rows, and every wave of the workgroup substitutes
+        // (the LDS tile keeps one wave busy while the other three wait).
+        // The accumulating last pass keeps the tile, which adds on store.
+        if (inner >= 64 && !(a == last && fused) && !(a == 0 && accum != nullptr)) sp.lines_per_block = 0;
         if (a == last && fused) {
             sp.src = values;
             sp.outer_dims = last;

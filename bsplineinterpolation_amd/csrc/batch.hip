@@ -11,9 +11,11 @@
 //
 // Launch structure of one build + eval:
 //   solve pass p (p = 0..2): the axis D-1-p of every spline with D > p,
-//     one persistent launch per (band width, cyclic) class over tiles of
-//     lines (batch_solve_kernel), the first pass reading the values with
-//     the periodic shift;
+//     one persistent launch (batch_solve_kernel) over work items of any
+//     band width, longest lines first: LDS tiles for contiguous axes (and
+//     axes with fewer than 64 adjacent lines), runs of 256 adjacent lines
+//     substituted in place for strided axes; the first pass reads the
+//     values with the periodic shift;
 //   eval: one persistent launch per (dimension, order) class over runs of
 //     points of one spline (batch_eval_kernel, batch_eval_d*.hip).
 #include <hip/hip_runtime.h>
@@ -39,18 +41,11 @@ namespace intp_amd {
 
 namespace {
 
-// Persistent: workgroup b solves items b, b + gridDim.x, ... (tiles of
-// lines of one spline along one axis, all of band width BW).
 template <int BW, bool CYC>
-__global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSpline* specs,
-                                                                   const BatchSolveItem* items, int32_t nitems,
-                                                                   const double* values, double* ctrl,
-                                                                   int32_t first_pass) {
-    extern __shared__ __align__(16) unsigned char tile_bytes[];
-    __shared__ int64_t dst_base[kMaxTileLines];
-    __shared__ int64_t src_base[kMaxTileLines];
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const BatchSolveItem w = items[it];
+__device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const BatchSolveItem& w, const double* values, double* ctrl,
+                                                 int32_t first_pass, unsigned char* tile_bytes, int64_t* dst_base,
+                                                 int64_t* src_base) {
+    if ((w.kind >> 4) == kSolveTile) {
         const BatchSpline& b = specs[w.spline];
         const int a = w.axis;
         SolveParams sp{};
@@ -76,35 +71,64 @@ __global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSp
             sp.shift_last = b.shift[a];
         }
         solve_tile<BW, CYC, double, double>(sp, w.first_line, tile_bytes, dst_base, src_base);
+        return;
+    }
+    // kSolveLanes: one line per lane, in place
+    const BatchSpline& b = specs[w.spline];
+    const int a = w.axis;
+    const int64_t n = b.ep.dim[a].n;
+    const int64_t line = w.first_line + threadIdx.x;
+    if (line >= b.total / n) return;
+    int64_t inner = 1;
+    for (int d = a + 1; d < b.dim; ++d) inner *= b.ep.dim[d].n;
+    SolveParams sp{};
+    sp.lower = b.lower[a];
+    sp.upper = b.upper[a];
+    sp.diag = b.diag[a];
+    sp.bottom = b.bottom[a];
+    sp.right = b.right[a];
+    const int64_t o = line / inner, ii = line - o * inner;
+    substitute<BW, CYC, double, double>(ctrl + b.offset + o * n * inner + ii, inner, n, sp);
+}
+
+// One launch per pass: workgroup g solves items g, g + gridDim.x, ...
+// (tiles of lines of one spline along one axis, of any band width; the
+// items are ordered longest line first, so the long sequential lines
+// start at once and the short tiles fill in around them).
+__global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSpline* specs,
+                                                                   const BatchSolveItem* items, int32_t nitems,
+                                                                   const double* values, double* ctrl,
+                                                                   int32_t first_pass) {
+    extern __shared__ __align__(16) unsigned char tile_bytes[];
+    __shared__ int64_t dst_base[kMaxTileLines];
+    __shared__ int64_t src_base[kMaxTileLines];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchSolveItem w = items[it];
+#define INTP_SOLVE_CASE(BW, CYC)                                                                                  \
+    case 2 * BW + CYC:                                                                                            \
+        batch_solve_item<BW, CYC>(specs, w, values, ctrl, first_pass, tile_bytes, dst_base, src_base);            \
+        break;
+        // periodic band widths are floor(k/2) <= 3, clamped ones k - 1 <= 6
+        switch (w.kind & 15) {
+            INTP_SOLVE_CASE(0, false)
+            INTP_SOLVE_CASE(0, true)
+            INTP_SOLVE_CASE(1, false)
+            INTP_SOLVE_CASE(1, true)
+            INTP_SOLVE_CASE(2, false)
+            INTP_SOLVE_CASE(2, true)
+            INTP_SOLVE_CASE(3, false)
+            INTP_SOLVE_CASE(3, true)
+            INTP_SOLVE_CASE(4, false)
+            INTP_SOLVE_CASE(5, false)
+            INTP_SOLVE_CASE(6, false)
+            default: break;  // kinds are checked on the host
+        }
+#undef INTP_SOLVE_CASE
         __syncthreads();
     }
 }
 
-template <int BW>
-void launch_solve_group(bool cyc, int blocks, size_t lds, hipStream_t s, const BatchSpline* specs,
-                        const BatchSolveItem* items, int32_t n, const double* values, double* ctrl, int first) {
-    if (cyc)
-        hipLaunchKernelGGL((batch_solve_kernel<BW, true>), dim3(blocks), dim3(kTileThreads), lds, s, specs, items, n,
-                           values, ctrl, first);
-    else
-        hipLaunchKernelGGL((batch_solve_kernel<BW, false>), dim3(blocks), dim3(kTileThreads), lds, s, specs, items,
-                           n, values, ctrl, first);
-}
-
-hipError_t launch_solve_bw(int bw, bool cyc, int blocks, size_t lds, hipStream_t s, const BatchSpline* specs,
-                           const BatchSolveItem* items, int32_t n, const double* values, double* ctrl, int first) {
-    switch (bw) {
-        case 0: launch_solve_group<0>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 1: launch_solve_group<1>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 2: launch_solve_group<2>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 3: launch_solve_group<3>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 4: launch_solve_group<4>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 5: launch_solve_group<5>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        case 6: launch_solve_group<6>(cyc, blocks, lds, s, specs, items, n, values, ctrl, first); break;
-        default: return hipErrorNotSupported;
-    }
-    return hipGetLastError();
-}
+constexpr int kMaxBatchBand = 6;
 
 // LDS budget of one solve workgroup (two per CU).
 constexpr int64_t kBatchTileBudget = 80 * 1024 - 2 * kMaxTileLines * 8;
@@ -117,8 +141,7 @@ int64_t tile_fit(int64_t n, int cap) {
 }
 
 struct SolveGroup {
-    int pass, bw;
-    bool cyc;
+    int pass;
     int64_t offset, count;
     size_t lds;
 };
@@ -193,37 +216,58 @@ namespace {
 // (Re)builds the solve work items for the current tuning; uploads them on
 // `stream` (the host copy stays alive in the batch).
 int make_solve_work(intp_batch b, hipStream_t stream) {
-    // per spline tile sizes
+    // per spline tile sizes (tiles serve the axes with fewer than 64
+    // adjacent lines)
     for (int s = 0; s < b->count; ++s) {
         BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
-        for (int d = 0; d < b->dims[static_cast<size_t>(s)]; ++d) {
+        const int D = b->dims[static_cast<size_t>(s)];
+        for (int d = 0; d < D; ++d) {
             const int64_t lpb = tile_fit(b->setups[static_cast<size_t>(s) * kMaxDim + d].n, b->tune.tile_cap);
-            if (lpb == 0) return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": line too long for a batch");
+            int64_t inner = 1;
+            for (int e = d + 1; e < D; ++e) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + e].n;
+            if (lpb == 0 && inner < 64)
+                return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": line too long for a batch");
             bs.lines_per_block[d] = static_cast<int32_t>(lpb);
         }
     }
     b->solve_host.clear();
     b->solve_groups.clear();
+    // a workgroup's time is about its longest line times the work per row
+    auto row_cost = [](const DimSetup& ds) { return int64_t{4} + ds.lu.p * (ds.lu.cyclic ? 4 : 2); };
+    auto kind_of = [](const DimSetup& ds, int mode) {
+        return static_cast<int16_t>((mode << 4) | (2 * ds.lu.p + (ds.lu.cyclic ? 1 : 0)));
+    };
+    std::vector<std::pair<int64_t, BatchSolveItem>> pass_items;
     for (int pass = 0; pass < kMaxDim; ++pass) {
-        for (int bw = 0; bw <= 6; ++bw)
-            for (int cyc = 0; cyc <= 1; ++cyc) {
-                SolveGroup g{pass, bw, cyc != 0, static_cast<int64_t>(b->solve_host.size()), 0, 0};
-                for (int s = 0; s < b->count; ++s) {
-                    const int D = b->dims[static_cast<size_t>(s)];
-                    if (D <= pass) continue;
-                    const int a = D - 1 - pass;
-                    const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + a];
-                    if (ds.lu.p != bw || ds.lu.cyclic != (cyc != 0)) continue;
-                    const BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
-                    const int64_t lpb = bs.lines_per_block[a];
-                    const int64_t nlines = bs.total / ds.n;
-                    for (int64_t first = 0; first < nlines; first += lpb)
-                        b->solve_host.push_back({s, a, first});
-                    g.lds = std::max(g.lds, static_cast<size_t>(lpb * (ds.n | 1) * 8));
-                }
-                g.count = static_cast<int64_t>(b->solve_host.size()) - g.offset;
-                if (g.count > 0) b->solve_groups.push_back(g);
+        SolveGroup g{pass, static_cast<int64_t>(b->solve_host.size()), 0, 0};
+        pass_items.clear();
+        for (int s = 0; s < b->count; ++s) {
+            const int D = b->dims[static_cast<size_t>(s)];
+            if (D <= pass) continue;
+            const int a = D - 1 - pass;
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + a];
+            if (ds.lu.p > (ds.lu.cyclic ? 3 : kMaxBatchBand))
+                return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
+            const BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
+            const int64_t nlines = bs.total / ds.n;
+            int64_t inner = 1;
+            for (int d = a + 1; d < D; ++d) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + d].n;
+            const int64_t cost = ds.n * row_cost(ds);
+            if (inner >= 64) {
+                for (int64_t first = 0; first < nlines; first += kTileThreads)
+                    pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveLanes), first}});
+            } else {
+                const int64_t lpb = bs.lines_per_block[a];
+                for (int64_t first = 0; first < nlines; first += lpb)
+                    pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveTile), first}});
+                g.lds = std::max(g.lds, static_cast<size_t>(lpb * (ds.n | 1) * 8));
             }
+        }
+        std::stable_sort(pass_items.begin(), pass_items.end(),
+                         [](const auto& x, const auto& y) { return x.first > y.first; });
+        for (const auto& pi : pass_items) b->solve_host.push_back(pi.second);
+        g.count = static_cast<int64_t>(b->solve_host.size()) - g.offset;
+        if (g.count > 0) b->solve_groups.push_back(g);
     }
     const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->solve_host.size()));
     if (need > b->solve_cap) {
@@ -287,10 +331,10 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
 int launch_build(intp_batch b, const double* values, hipStream_t stream) {
     for (const SolveGroup& g : b->solve_groups) {
         const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
-        const hipError_t e = launch_solve_bw(g.bw, g.cyc, blocks, g.lds, stream, b->specs, b->solve_items + g.offset,
-                                             static_cast<int32_t>(g.count), values, b->ctrl, g.pass == 0 ? 1 : 0);
-        if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
-        HIP_TRY(e, "batch solve launch");
+        hipLaunchKernelGGL(batch_solve_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kTileThreads), g.lds, stream,
+                           b->specs, b->solve_items + g.offset, static_cast<int32_t>(g.count), values, b->ctrl,
+                           g.pass == 0 ? 1 : 0);
+        HIP_TRY(hipGetLastError(), "batch solve launch");
     }
     return INTP_OK;
 }

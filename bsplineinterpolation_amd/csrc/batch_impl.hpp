@@ -26,10 +26,19 @@ struct BatchSpline {
     int32_t lines_per_block[kMaxDim];  // LDS tile of the solve along each axis
 };
 
-// A tile of lines of one spline along one axis.
+// Work of one solve workgroup.  kind & 15 = 2 * band width + cyclic
+// selects the substitution; kind >> 4 the mode:
+//   kSolveTile: lines first_line .. of `spline` along `axis`, staged
+//     through LDS (contiguous axes, and axes with fewer than 64 adjacent
+//     lines);
+//   kSolveLanes: 256 adjacent lines first_line .. of `spline` along a
+//     strided `axis`, one per lane, substituted in place.
+enum : int16_t { kSolveTile = 0, kSolveLanes = 1 };
+
 struct BatchSolveItem {
     int32_t spline;
-    int32_t axis;
+    int16_t axis;
+    int16_t kind;
     int64_t first_line;
 };
 

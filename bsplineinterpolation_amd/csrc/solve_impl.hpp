@@ -4,14 +4,14 @@
 // factorization (InterpolationTemplate.hpp:448-580).  One lane solves one
 // line.
 //
-// Main kernel (solve_tile_kernel): a workgroup copies the lines of up to
-// 64 neighbouring lanes into LDS with coalesced bulk loads (runs of one
-// line for the contiguous last axis, runs across lines for the strided
-// axes), each lane substitutes its line in LDS, and the workgroup writes
+// Contiguous axis (solve_tile_kernel): a workgroup copies the lines of up
+// to 64 neighbouring lanes into LDS with coalesced bulk loads (runs of one
+// line), each lane substitutes its line in LDS, and the workgroup writes
 // the tile back the same way.  The first pass reads the caller's values
 // with the periodic index shift applied (InterpolationTemplate.hpp:452-462),
-// so no separate copy precedes it.  Lines too long for LDS fall back to
-// solve_kernel, which substitutes in place in HBM.
+// so no separate copy precedes it.  Strided axes with 64 or more adjacent
+// lines, and lines too long for LDS, use solve_kernel, which substitutes
+// in place in HBM (coalesced across lanes on a strided axis).
 //
 // The substitution is the reference's (BandLU.hpp:121-143, 216-259)
 // reordered by rows: each entry receives the same updates in the same
@@ -48,90 +48,126 @@ struct LineState {
     C corner[BW + 1];  // backward: x[n-BW+c]
 };
 
-// Forward substitution of rows j0 .. j0+B-1 (unit lower triangle, plus
-// the bottom strip of a cyclic matrix).
-template <int BW, bool CYC, typename C, typename P, int B>
-__device__ __forceinline__ void forward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
-                                             const SolveParams& sp, LineState<BW, CYC, C>& st) {
-#pragma clang fp contract(off)
-    const C* __restrict__ lower = static_cast<const C*>(sp.lower);
-    const C* __restrict__ bottom = static_cast<const C*>(sp.bottom);
+// A zero the compiler cannot see through, held in a vector register.  The
+// factor coefficients are the same for every lane, so without it they
+// would be read with scalar loads, whose completion counter is shared with
+// LDS reads and not ordered: every batch would then wait for all loads in
+// flight, including the next batch's.  Offset by this zero they are
+// vector loads, which complete in order, so the next batch stays in
+// flight while the current one is substituted.
+__device__ __forceinline__ int32_t lane_zero() {
+    int32_t z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return z;
+}
+
+// Registers of one batch of forward rows: the loaded right-hand sides and
+// the factor coefficients of those rows.
+template <int BW, bool CYC, typename C, int B>
+struct ForwardBatch {
     C xb[B];
     C lc[B][BW + 1];
     C sc[B][BW + 1];
+};
+
+// Registers of one batch of backward rows.
+template <int BW, bool CYC, typename C, int B>
+struct BackwardBatch {
+    C xb[B];
+    C uc[B][BW + 1];
+    C rc[B][BW + 1];
+    C dg[B];
+};
+
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void load_forward(ForwardBatch<BW, CYC, C, B>& r, const P* __restrict__ x, int64_t step,
+                                             int64_t n, int64_t j0, const SolveParams& sp, int32_t z) {
+    const C* __restrict__ lower = static_cast<const C*>(sp.lower) + z;
+    const C* __restrict__ bottom = static_cast<const C*>(sp.bottom) + z;
 #pragma unroll
     for (int u = 0; u < B; ++u) {
-        xb[u] = static_cast<C>(x[(j0 + u) * step]);
+        r.xb[u] = static_cast<C>(x[(j0 + u) * step]);
 #pragma unroll
-        for (int m = 0; m < BW; ++m) lc[u][m] = lower[(j0 + u) * BW + m];
+        for (int m = 0; m < BW; ++m) r.lc[u][m] = lower[(j0 + u) * BW + m];
         if constexpr (CYC) {
 #pragma unroll
-            for (int r = 0; r < BW; ++r) sc[u][r] = bottom[r * n + j0 + u];
+            for (int q = 0; q < BW; ++q) r.sc[u][q] = bottom[q * n + j0 + u];
         }
     }
+}
+
+// Forward substitution of rows j0 .. j0+B-1 (unit lower triangle, plus
+// the bottom strip of a cyclic matrix) from loaded registers; stores the
+// rows.
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void run_forward(ForwardBatch<BW, CYC, C, B>& r, P* __restrict__ x, int64_t step,
+                                            int64_t n, int64_t j0, LineState<BW, CYC, C>& st) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 + u;
-        C v = xb[u];
+        C v = r.xb[u];
         if constexpr (CYC) {
 #pragma unroll
-            for (int r = 0; r < BW; ++r)
-                if (j == n - BW + r) v = st.acc[r];
+            for (int q = 0; q < BW; ++q)
+                if (j == n - BW + q) v = st.acc[q];
         }
 #pragma unroll
-        for (int m = BW; m >= 1; --m) v = v - lc[u][m - 1] * st.hist[m - 1];
-        xb[u] = v;
+        for (int m = BW; m >= 1; --m) v = v - r.lc[u][m - 1] * st.hist[m - 1];
+        r.xb[u] = v;
 #pragma unroll
         for (int m = BW; m >= 2; --m) st.hist[m - 1] = st.hist[m - 2];
         if (BW > 0) st.hist[0] = v;
         if constexpr (CYC) {
 #pragma unroll
-            for (int r = 0; r < BW; ++r)
-                if (j < n - BW + r - BW) st.acc[r] = st.acc[r] - sc[u][r] * v;
+            for (int q = 0; q < BW; ++q)
+                if (j < n - BW + q - BW) st.acc[q] = st.acc[q] - r.sc[u][q] * v;
         }
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u) x[(j0 + u) * step] = static_cast<P>(xb[u]);
+    for (int u = 0; u < B; ++u) x[(j0 + u) * step] = static_cast<P>(r.xb[u]);
+}
+
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void load_backward(BackwardBatch<BW, CYC, C, B>& r, const P* __restrict__ x, int64_t step,
+                                              int64_t n, int64_t j0, const SolveParams& sp, int32_t z) {
+    const C* __restrict__ upper = static_cast<const C*>(sp.upper) + z;
+    const C* __restrict__ diag = static_cast<const C*>(sp.diag) + z;
+    const C* __restrict__ right = static_cast<const C*>(sp.right) + z;
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        const int64_t j = j0 - u;
+        r.xb[u] = static_cast<C>(x[j * step]);
+#pragma unroll
+        for (int m = 0; m < BW; ++m) r.uc[u][m] = upper[j * BW + m];
+        r.dg[u] = diag[j];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int c = 0; c < BW; ++c) r.rc[u][c] = right[c * n + j];
+        }
+    }
 }
 
 // Backward substitution of rows j0, j0-1, .., j0-B+1 (upper triangle,
-// plus the right strip of a cyclic matrix).
+// plus the right strip of a cyclic matrix) from loaded registers; stores
+// the rows.
 template <int BW, bool CYC, typename C, typename P, int B>
-__device__ __forceinline__ void backward_rows(P* __restrict__ x, int64_t step, int64_t n, int64_t j0,
-                                              const SolveParams& sp, LineState<BW, CYC, C>& st) {
+__device__ __forceinline__ void run_backward(BackwardBatch<BW, CYC, C, B>& r, P* __restrict__ x, int64_t step,
+                                             int64_t n, int64_t j0, LineState<BW, CYC, C>& st) {
 #pragma clang fp contract(off)
-    const C* __restrict__ upper = static_cast<const C*>(sp.upper);
-    const C* __restrict__ diag = static_cast<const C*>(sp.diag);
-    const C* __restrict__ right = static_cast<const C*>(sp.right);
-    C xb[B];
-    C uc[B][BW + 1];
-    C rc[B][BW + 1];
-    C dg[B];
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 - u;
-        xb[u] = static_cast<C>(x[j * step]);
-#pragma unroll
-        for (int m = 0; m < BW; ++m) uc[u][m] = upper[j * BW + m];
-        dg[u] = diag[j];
-        if constexpr (CYC) {
-#pragma unroll
-            for (int c = 0; c < BW; ++c) rc[u][c] = right[c * n + j];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < B; ++u) {
-        const int64_t j = j0 - u;
-        C v = xb[u];
+        C v = r.xb[u];
         if constexpr (CYC) {
 #pragma unroll
             for (int c = BW - 1; c >= 0; --c)
-                if (j < n - BW + c - BW) v = v - rc[u][c] * st.corner[c];
+                if (j < n - BW + c - BW) v = v - r.rc[u][c] * st.corner[c];
         }
 #pragma unroll
-        for (int m = BW; m >= 1; --m) v = v - uc[u][m - 1] * st.nxt[m - 1];
-        v = v / dg[u];
-        xb[u] = v;
+        for (int m = BW; m >= 1; --m) v = v - r.uc[u][m - 1] * st.nxt[m - 1];
+        v = v / r.dg[u];
+        r.xb[u] = v;
 #pragma unroll
         for (int m = BW; m >= 2; --m) st.nxt[m - 1] = st.nxt[m - 2];
         if (BW > 0) st.nxt[0] = v;
@@ -142,26 +178,69 @@ __device__ __forceinline__ void backward_rows(P* __restrict__ x, int64_t step, i
         }
     }
 #pragma unroll
-    for (int u = 0; u < B; ++u) x[(j0 - u) * step] = static_cast<P>(xb[u]);
+    for (int u = 0; u < B; ++u) x[(j0 - u) * step] = static_cast<P>(r.xb[u]);
+}
+
+// Rows per batch: two batches are in registers at once (the one being
+// substituted and the next one loading), so wide cyclic bands use 4.
+template <int BW, bool CYC>
+constexpr int batch_rows() {
+    return BW * (CYC ? 2 : 1) <= 4 ? kBatch : kBatch / 2;
 }
 
 // Solves one line x[0], x[step], ..., x[(n-1)*step] in place, computing
-// in C (elements of x have type P).
+// in C (elements of x have type P).  Software-pipelined: the loads of the next batch of rows (right-hand
+// sides and factor coefficients) are in flight while the current batch is
+// substituted, so one lane's chain of dependent rows does not wait for
+// memory every batch.  Each row receives the same updates in the same
+// order as without pipelining.
 template <int BW, bool CYC, typename C, typename P>
 __device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
+    constexpr int B = batch_rows<BW, CYC>();
     LineState<BW, CYC, C> st;
 #pragma unroll
     for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = C(0);
     if constexpr (CYC) {
 #pragma unroll
-        for (int r = 0; r < BW; ++r) st.acc[r] = static_cast<C>(x[(n - BW + r) * step]);
+        for (int q = 0; q < BW; ++q) st.acc[q] = static_cast<C>(x[(n - BW + q) * step]);
     }
-    int64_t j = 0;
-    for (; j + kBatch <= n; j += kBatch) forward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
-    for (; j < n; ++j) forward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
-    j = n - 1;
-    for (; j - kBatch + 1 >= 0; j -= kBatch) backward_rows<BW, CYC, C, P, kBatch>(x, step, n, j, sp, st);
-    for (; j >= 0; --j) backward_rows<BW, CYC, C, P, 1>(x, step, n, j, sp, st);
+    const int32_t z = lane_zero();
+    const int64_t nb = n / B;  // full batches
+    if (nb > 0) {
+        ForwardBatch<BW, CYC, C, B> a, b;
+        load_forward<BW, CYC, C, P, B>(a, x, step, n, 0, sp, z);
+        int64_t t = 0;
+        for (; t + 2 <= nb; t += 2) {
+            load_forward<BW, CYC, C, P, B>(b, x, step, n, (t + 1) * B, sp, z);
+            run_forward<BW, CYC, C, P, B>(a, x, step, n, t * B, st);
+            // batch t + 2, or (past the end) a harmless reload of batch nb - 1
+            load_forward<BW, CYC, C, P, B>(a, x, step, n, min(t + 2, nb - 1) * B, sp, z);
+            run_forward<BW, CYC, C, P, B>(b, x, step, n, (t + 1) * B, st);
+        }
+        if (t < nb) run_forward<BW, CYC, C, P, B>(a, x, step, n, t * B, st);
+    }
+    for (int64_t j = nb * B; j < n; ++j) {
+        ForwardBatch<BW, CYC, C, 1> r;
+        load_forward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
+        run_forward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
+    }
+    if (nb > 0) {
+        BackwardBatch<BW, CYC, C, B> a, b;
+        load_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1, sp, z);
+        int64_t t = 0;
+        for (; t + 2 <= nb; t += 2) {
+            load_backward<BW, CYC, C, P, B>(b, x, step, n, n - 1 - (t + 1) * B, sp, z);
+            run_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - t * B, st);
+            load_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - min(t + 2, nb - 1) * B, sp, z);
+            run_backward<BW, CYC, C, P, B>(b, x, step, n, n - 1 - (t + 1) * B, st);
+        }
+        if (t < nb) run_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - t * B, st);
+    }
+    for (int64_t j = n - 1 - nb * B; j >= 0; --j) {
+        BackwardBatch<BW, CYC, C, 1> r;
+        load_backward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
+        run_backward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
+    }
 }
 
 // Fallback: one lane per line, substituting in HBM (storage T, compute C;
