@@ -401,3 +401,115 @@ class Batch:
         out = np.empty(n)
         check(_lib.load().intp_batch_download_ctrl(self.h, int(s), out.ctypes.data_as(_dp), n))
         return out
+
+
+class Server:
+    """The serving loop (intp_server_* / intp_serve_*): builds and
+    evaluations are queued from any thread and run by the library's
+    dispatcher thread on ``streams`` HIP streams; builds are admitted
+    against ``hbm_budget`` bytes of resident control points, evicting
+    least-recently-used idle splines.  Request ids returned by ``build``
+    also name the spline for ``eval``.  Tensors passed to a request must
+    stay alive until it completes (the server keeps a reference until its
+    ``wait``/``drain``)."""
+
+    def __init__(self, ctx: Optional[Context] = None, hbm_budget: int = 0, streams: int = 4,
+                 max_inflight: int = 256):
+        lib = _lib.load()
+        self.ctx = ctx or Context.default()
+        cfg = _lib.ServerConfig(int(hbm_budget), int(streams), int(max_inflight))
+        h = C.c_void_p()
+        check(lib.intp_server_create(self.ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._hold = {}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+            _lib._LIB.intp_server_destroy(self.h)
+            self.h = None
+            self._hold.clear()
+
+    def __del__(self):
+        self.close()
+
+    def build(self, order: int, values, periodic=None, ranges=None, coords=None, precision: str = "f64",
+              sweeps: int = 2) -> int:
+        """Queues a build of device tensor ``values`` (float64, C-order);
+        returns the request (= spline) id."""
+        desc, keep = _grid_desc(order, tuple(values.shape), periodic, ranges, coords)
+        assert _is_torch(values) and str(values.dtype) == "torch.float64", "values must be a float64 device tensor"
+        values = values.contiguous()
+        prec = _lib.INTP_PREC_F32_REFINED if _refined(precision) else _lib.INTP_PREC_F64
+        rid = C.c_int64()
+        check(_lib.load().intp_serve_build(self.h, C.byref(desc), C.cast(values.data_ptr(), _dp), prec, int(sweeps),
+                                           C.byref(rid)))
+        del keep
+        self._hold[rid.value] = values
+        return rid.value
+
+    def eval(self, spline: int, coords, orders=None, out=None, out_dtype=None) -> int:
+        """Queues an evaluation of ``coords`` ([npts, D] device tensor, or
+        [npts, 3] with the trailing columns ignored) for derivative
+        ``orders`` (list of D-tuples; default value only) into ``out``
+        ([nout, npts] device tensor, allocated if None; read it after
+        ``wait``).  Returns the request id."""
+        import torch
+        if not orders:
+            raise ValueError("orders must be given (one tuple of D derivative orders per output)")
+        npts, cols = int(coords.shape[0]), int(coords.shape[1])
+        coords = coords.contiguous()
+        dim = len(orders[0])
+        ords = np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1))
+        nout = len(orders)
+        f32 = out_dtype is not None and str(out_dtype) == "torch.float32"
+        if out is None:
+            out = torch.empty((nout, npts), dtype=torch.float32 if f32 else torch.float64, device=coords.device)
+        flags = _lib.INTP_EVAL_OUT_F32 if str(out.dtype) == "torch.float32" else 0
+        assert dim <= cols
+        rid = C.c_int64()
+        check(_lib.load().intp_serve_eval(self.h, int(spline), npts, C.cast(coords.data_ptr(), _dp), 1, cols, nout,
+                                          ords.ctypes.data_as(C.POINTER(C.c_int32)), C.cast(out.data_ptr(), _dp),
+                                          npts, 1, flags, C.byref(rid)))
+        self._hold[rid.value] = (coords, out)
+        return rid.value
+
+    def poll(self, request: int):
+        """(state, status): state is one of INTP_REQ_QUEUED/RUNNING/DONE/FAILED."""
+        st, rc = C.c_int32(), C.c_int32()
+        check(_lib.load().intp_serve_poll(self.h, int(request), C.byref(st), C.byref(rc)))
+        return st.value, rc.value
+
+    def wait(self, request: int):
+        """Blocks until the request finishes; raises if it failed.  Returns
+        the output tensor of an evaluation (None for a build)."""
+        held = self._hold.pop(int(request), None)
+        check(_lib.load().intp_serve_wait(self.h, int(request)))
+        return held[1] if isinstance(held, tuple) else None
+
+    def drain(self):
+        """Blocks until every submitted request has finished."""
+        check(_lib.load().intp_serve_drain(self.h))
+        keep = {k: v for k, v in self._hold.items() if not isinstance(v, tuple)}
+        self._hold = keep  # build inputs stay referenced by their ids until released
+
+    def release(self, spline: int):
+        check(_lib.load().intp_serve_release(self.h, int(spline)))
+        self._hold.pop(int(spline), None)
+
+    def spline(self, spline: int) -> InterpolationFunction:
+        """The built spline as an InterpolationFunction (a new reference)."""
+        h = C.c_void_p()
+        check(_lib.load().intp_serve_spline(self.h, int(spline), C.byref(h)))
+        return InterpolationFunction._wrap(h, self.ctx)
+
+    def stats(self, reset: bool = False) -> dict:
+        c = _lib.ServeCounters()
+        check(_lib.load().intp_serve_stats(self.h, 1 if reset else 0, C.byref(c)))
+        out = {}
+        for k, name in ((_lib.INTP_REQ_BUILD, "build"), (_lib.INTP_REQ_EVAL, "eval")):
+            t = c.type[k]
+            out[name] = dict(completed=t.completed, failed=t.failed, bytes=t.bytes, p50_ms=t.latency_p50_ms,
+                             p99_ms=t.latency_p99_ms, mean_ms=t.latency_mean_ms, max_ms=t.latency_max_ms)
+        out.update(resident_splines=c.resident_splines, resident_bytes=c.resident_bytes, evictions=c.evictions,
+                   admission_waits=c.admission_waits, first_submit_s=c.first_submit_s, last_done_s=c.last_done_s)
+        return out

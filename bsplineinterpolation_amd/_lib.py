@@ -20,6 +20,7 @@ INTP_E_DOMAIN = 2
 INTP_E_NOMEM = 3
 INTP_E_HIP = 4
 INTP_E_UNSUPPORTED = 5
+INTP_E_EVICTED = 6
 
 INTP_MEM_DEVICE = 0
 INTP_MEM_HOST = 1
@@ -30,6 +31,14 @@ INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
 INTP_BATCH_GRAPH = 1
 INTP_BATCH_AUTOTUNE = 2
+INTP_REQ_BUILD = 0
+INTP_REQ_EVAL = 1
+INTP_REQ_QUEUED = 0
+INTP_REQ_RUNNING = 1
+INTP_REQ_DONE = 2
+INTP_REQ_FAILED = 3
+INTP_PREC_F64 = 0
+INTP_PREC_F32_REFINED = 1
 
 # every symbol include/intp_amd.h declares
 EXPORTS = (
@@ -40,6 +49,8 @@ EXPORTS = (
     "intp_interpolate_refined", "intp_build_refined",
     "intp_batch_create", "intp_batch_destroy", "intp_batch_info", "intp_batch_build", "intp_batch_eval",
     "intp_batch_run", "intp_batch_tuning", "intp_batch_download_ctrl",
+    "intp_server_create", "intp_server_destroy", "intp_serve_build", "intp_serve_eval", "intp_serve_poll",
+    "intp_serve_wait", "intp_serve_drain", "intp_serve_release", "intp_serve_spline", "intp_serve_stats",
 )
 
 
@@ -54,6 +65,22 @@ class SplineInfo(C.Structure):
                 ("uniform", C.c_int32 * MAX_DIM), ("n", C.c_int64 * MAX_DIM), ("nknots", C.c_int64 * MAX_DIM),
                 ("range_lo", C.c_double * MAX_DIM), ("range_hi", C.c_double * MAX_DIM),
                 ("ctrl_bytes", C.c_int32), ("refine_sweeps", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("streams", C.c_int32), ("max_inflight", C.c_int32)]
+
+
+class ServeTypeCounters(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("bytes", C.c_double),
+                ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_mean_ms", C.c_double), ("latency_max_ms", C.c_double)]
+
+
+class ServeCounters(C.Structure):
+    _fields_ = [("type", ServeTypeCounters * 2), ("resident_splines", C.c_int64), ("resident_bytes", C.c_int64),
+                ("evictions", C.c_int64), ("admission_waits", C.c_int64), ("first_submit_s", C.c_double),
+                ("last_done_s", C.c_double)]
 
 
 _LIB = None
@@ -106,6 +133,17 @@ def load(path: str = LIB_PATH):
     lib.intp_batch_run.argtypes = [vp, dp, dp, i64p, dp, C.c_uint32, vp]
     lib.intp_batch_tuning.argtypes = [vp, C.c_char_p, C.c_int64]
     lib.intp_batch_download_ctrl.argtypes = [vp, C.c_int32, dp, C.c_int64]
+    lib.intp_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+    lib.intp_server_destroy.argtypes = [vp]
+    lib.intp_serve_build.argtypes = [vp, C.POINTER(GridDesc), dp, C.c_int32, C.c_int32, i64p]
+    lib.intp_serve_eval.argtypes = [vp, C.c_int64, C.c_int64, dp, C.c_int64, C.c_int64, C.c_int32, i32p, dp,
+                                    C.c_int64, C.c_int64, C.c_uint32, i64p]
+    lib.intp_serve_poll.argtypes = [vp, C.c_int64, i32p, i32p]
+    lib.intp_serve_wait.argtypes = [vp, C.c_int64]
+    lib.intp_serve_drain.argtypes = [vp]
+    lib.intp_serve_release.argtypes = [vp, C.c_int64]
+    lib.intp_serve_spline.argtypes = [vp, C.c_int64, C.POINTER(vp)]
+    lib.intp_serve_stats.argtypes = [vp, C.c_int32, C.POINTER(ServeCounters)]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libintp_amd.so")
 OBJ = os.path.join(PKG, "_build")
 ARCH = os.environ.get("INTP_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["abi.hip", "batch.hip", "bin.hip", "refine.hip", "solve.hip"] + [
+HIP_SOURCES = ["abi.hip", "batch.hip", "bin.hip", "refine.hip", "serve.hip", "solve.hip"] + [
     "batch_eval_d%d.hip" % d for d in (1, 2, 3)] + [
     "eval_%s_d%d.hip" % (t, d) for t in ("f64", "f32") for d in (1, 2, 3)]
 CXX_SOURCES = ["setup.cpp"]

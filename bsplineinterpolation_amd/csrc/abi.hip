@@ -440,6 +440,7 @@ const char* intp_status_string(int status) {
         case INTP_E_NOMEM: return "out of device memory";
         case INTP_E_HIP: return "HIP runtime error";
         case INTP_E_UNSUPPORTED: return "unsupported dimension or order";
+        case INTP_E_EVICTED: return "spline not resident";
         default: return "unknown status";
     }
 }

@@ -1,0 +1,610 @@
+// serve.hip -- the serving loop (intp_server_* / intp_serve_*): builds and
+// evaluations submitted by any number of client threads, admitted by a
+// dispatcher thread against an HBM budget, run on several HIP streams,
+// completed through HIP events.
+//
+// Dispatcher turn (continuous batching: a turn never waits for the work of
+// an earlier turn):
+//   1. take every queued request, oldest first;
+//      eval  -> launched on the next stream, after a stream wait on its
+//               spline's build event; fails with INTP_E_EVICTED if the
+//               spline is gone;
+//      build -> admitted if its control points fit the budget beside the
+//               resident splines, after evicting least-recently-used idle
+//               splines if needed; otherwise it stays queued for a later
+//               turn (evaluations behind it are not held up);
+//   2. poll the events of launched requests; a finished request gets its
+//      completion time, unpins its spline, and wakes waiting clients;
+//   3. sleep until a submission, or briefly while work is in flight.
+// Builds and evaluations go through intp_build / intp_build_refined /
+// intp_eval, so their results equal those calls' on the same inputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/intp_amd.h"
+#include "runtime.hpp"
+
+using namespace intp_amd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kMaxServeStreams = 16;
+
+struct Request {
+    int kind = INTP_REQ_BUILD;
+    int state = INTP_REQ_QUEUED;
+    int status = INTP_OK;
+    double t_submit = 0, t_done = 0;  // server clock, s
+    double bytes = 0;                 // algorithmic bytes
+    int64_t spline = -1;              // build: its own id; eval: the target
+    // build
+    intp_grid_desc desc{};
+    std::vector<std::vector<double>> coords_copy;  // non-uniform node coordinates
+    const double* values = nullptr;
+    int precision = INTP_PREC_F64;
+    int sweeps = 0;
+    int64_t footprint = 0;
+    bool waited = false;  // held back for memory at least once
+    // eval
+    int64_t npts = 0;
+    const double* coords = nullptr;
+    int64_t cs_dim = 0, cs_pt = 0;
+    std::vector<int32_t> orders;
+    int32_t nout = 0;
+    double* out = nullptr;
+    int64_t os_out = 0, os_pt = 0;
+    uint32_t flags = 0;
+    // running
+    hipEvent_t done = nullptr;
+};
+
+struct Resident {
+    intp_spline h = nullptr;
+    int64_t bytes = 0;
+    int pins = 0;         // launched, unfinished evaluations
+    bool ready = false;   // build finished
+    bool released = false;
+    hipEvent_t built = nullptr;
+    std::list<int64_t>::iterator lru;
+};
+
+// Control-point bytes a build keeps resident, plus what a refined build
+// needs while it runs (FP32 correction and two FP64 residual buffers).
+int64_t build_footprint(const intp_grid_desc& d, int precision) {
+    int64_t n = 1;
+    for (int i = 0; i < d.dim; ++i) n *= d.n[i];
+    return precision == INTP_PREC_F32_REFINED ? n * 4 + n * (4 + 16) : n * 8;
+}
+
+int64_t resident_bytes_of(const intp_grid_desc& d, int precision) {
+    int64_t n = 1;
+    for (int i = 0; i < d.dim; ++i) n *= d.n[i];
+    return n * (precision == INTP_PREC_F32_REFINED ? 4 : 8);
+}
+
+// Algorithmic HBM bytes of a build (2 D N s per solve; refined: S+1 FP32
+// solves, S FP64 residuals with the subtraction of f, S FP32 updates).
+double build_bytes(const intp_grid_desc& d, int precision, int sweeps) {
+    double n = 1;
+    for (int i = 0; i < d.dim; ++i) n *= static_cast<double>(d.n[i]);
+    const double D = d.dim;
+    if (precision != INTP_PREC_F32_REFINED) return 2 * D * n * 8;
+    const double S = sweeps;
+    return (S + 1) * 2 * D * n * 4 + S * (2 * D * n * 8 + 8 * n) + S * n * 4;
+}
+
+}  // namespace
+
+struct intp_server_s {
+    int device = 0;
+    intp_ctx ctx = nullptr;
+    int64_t budget = 0;
+    int max_inflight = 256;
+    std::vector<hipStream_t> streams;
+    size_t next_stream = 0;
+    Clock::time_point t0 = Clock::now();
+
+    std::mutex mu;
+    std::condition_variable wake;  // dispatcher: new work
+    std::condition_variable done;  // clients: a request finished
+    bool stop = false;
+    int64_t next_id = 1;
+    std::deque<int64_t> queue;
+    std::unordered_map<int64_t, Request> reqs;
+    std::unordered_map<int64_t, Resident> splines;
+    std::list<int64_t> lru;  // front: least recently used
+    int64_t resident = 0;
+    std::vector<int64_t> inflight;
+    int64_t unfinished = 0;  // submitted, not yet done or failed
+    std::vector<hipEvent_t> spare_events;
+
+    // statistics of the current period
+    std::vector<double> lat[2];
+    double bytes[2] = {0, 0};
+    int64_t failed[2] = {0, 0};
+    int64_t evictions = 0, admission_waits = 0;
+    double first_submit = -1, last_done = 0;
+
+    std::thread dispatcher;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+    void run();
+    void finish(int64_t id, Request& r, int status);
+    bool admit_build(int64_t id, Request& r);
+    bool launch_eval(int64_t id, Request& r);
+    void poll_inflight();
+    void evict_idle_released();
+    hipEvent_t take_event();
+};
+
+hipEvent_t intp_server_s::take_event() {
+    if (!spare_events.empty()) {
+        hipEvent_t e = spare_events.back();
+        spare_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Marks a request finished (caller holds mu).
+void intp_server_s::finish(int64_t id, Request& r, int status) {
+    r.status = status;
+    r.state = status == INTP_OK ? INTP_REQ_DONE : INTP_REQ_FAILED;
+    r.t_done = now();
+    if (status == INTP_OK) {
+        lat[r.kind].push_back((r.t_done - r.t_submit) * 1e3);
+        bytes[r.kind] += r.bytes;
+        last_done = std::max(last_done, r.t_done);
+    } else {
+        ++failed[r.kind];
+    }
+    if (r.done) {
+        spare_events.push_back(r.done);
+        r.done = nullptr;
+    }
+    (void)id;
+    --unfinished;
+    done.notify_all();
+}
+
+// Frees released splines that nothing uses any more (caller holds mu).
+void intp_server_s::evict_idle_released() {
+    for (auto it = splines.begin(); it != splines.end();) {
+        Resident& s = it->second;
+        if (s.released && s.ready && s.pins == 0) {
+            resident -= s.bytes;
+            lru.erase(s.lru);
+            if (s.built) spare_events.push_back(s.built);
+            (void)intp_spline_release(s.h);
+            it = splines.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// Admits and launches a build if it fits the budget (caller holds mu).
+bool intp_server_s::admit_build(int64_t id, Request& r) {
+    if (r.footprint > budget) {
+        finish(id, r, fail(INTP_E_NOMEM, "build larger than the server's HBM budget"));
+        return true;
+    }
+    // evict least-recently-used idle splines until it fits
+    for (auto it = lru.begin(); resident + r.footprint > budget && it != lru.end();) {
+        const int64_t victim = *it;
+        Resident& s = splines[victim];
+        if (s.ready && s.pins == 0) {
+            resident -= s.bytes;
+            if (s.built) spare_events.push_back(s.built);
+            (void)intp_spline_release(s.h);
+            it = lru.erase(it);
+            splines.erase(victim);
+            ++evictions;
+        } else {
+            ++it;
+        }
+    }
+    if (resident + r.footprint > budget) {
+        if (!r.waited) ++admission_waits;
+        r.waited = true;
+        return false;  // wait for running work to finish
+    }
+    hipStream_t st = streams[next_stream++ % streams.size()];
+    intp_spline h = nullptr;
+    const int rc = r.precision == INTP_PREC_F32_REFINED
+                       ? intp_build_refined(ctx, &r.desc, r.values, INTP_MEM_DEVICE, r.sweeps, &h, st)
+                       : intp_build(ctx, &r.desc, r.values, INTP_MEM_DEVICE, &h, st);
+    if (rc != INTP_OK) {
+        finish(id, r, rc);
+        return true;
+    }
+    hipEvent_t ev = take_event();
+    hipEvent_t built = take_event();
+    if (!ev || !built || hipEventRecord(ev, st) != hipSuccess || hipEventRecord(built, st) != hipSuccess) {
+        (void)intp_spline_release(h);
+        if (ev) spare_events.push_back(ev);
+        if (built) spare_events.push_back(built);
+        finish(id, r, fail(INTP_E_HIP, "hipEventRecord"));
+        return true;
+    }
+    Resident s;
+    s.h = h;
+    s.bytes = resident_bytes_of(r.desc, r.precision);
+    s.built = built;
+    lru.push_back(id);
+    s.lru = std::prev(lru.end());
+    splines.emplace(id, s);
+    resident += s.bytes;
+    r.done = ev;
+    r.state = INTP_REQ_RUNNING;
+    inflight.push_back(id);
+    return true;
+}
+
+// Launches an evaluation (caller holds mu).
+bool intp_server_s::launch_eval(int64_t id, Request& r) {
+    auto it = splines.find(r.spline);
+    if (it == splines.end() || it->second.released) {
+        finish(id, r, fail(INTP_E_EVICTED, "spline " + std::to_string(r.spline) + " is not resident"));
+        return true;
+    }
+    Resident& s = it->second;
+    hipStream_t st = streams[next_stream++ % streams.size()];
+    if (!s.ready && hipStreamWaitEvent(st, s.built, 0) != hipSuccess) {
+        finish(id, r, fail(INTP_E_HIP, "hipStreamWaitEvent"));
+        return true;
+    }
+    const int rc = intp_eval(s.h, r.npts, r.coords, r.cs_dim, r.cs_pt, r.nout, r.orders.data(), r.out, r.os_out,
+                             r.os_pt, r.flags, st);
+    if (rc != INTP_OK) {
+        finish(id, r, rc);
+        return true;
+    }
+    hipEvent_t ev = take_event();
+    if (!ev || hipEventRecord(ev, st) != hipSuccess) {
+        if (ev) spare_events.push_back(ev);
+        (void)hipStreamSynchronize(st);  // the launch is in flight; finish it before reporting
+        finish(id, r, fail(INTP_E_HIP, "hipEventRecord"));
+        return true;
+    }
+    ++s.pins;
+    lru.splice(lru.end(), lru, s.lru);  // most recently used
+    r.done = ev;
+    r.state = INTP_REQ_RUNNING;
+    inflight.push_back(id);
+    return true;
+}
+
+// Completes launched requests whose events have fired (caller holds mu).
+void intp_server_s::poll_inflight() {
+    size_t keep = 0;
+    for (size_t i = 0; i < inflight.size(); ++i) {
+        const int64_t id = inflight[i];
+        Request& r = reqs[id];
+        const hipError_t q = hipEventQuery(r.done);
+        if (q == hipErrorNotReady) {
+            inflight[keep++] = id;
+            continue;
+        }
+        auto it = splines.find(r.spline);
+        if (r.kind == INTP_REQ_EVAL) {
+            if (it != splines.end()) --it->second.pins;
+        } else if (it != splines.end()) {
+            it->second.ready = true;
+        }
+        finish(id, r, q == hipSuccess ? INTP_OK : hip_fail(q, "request"));
+    }
+    inflight.resize(keep);
+    evict_idle_released();
+}
+
+void intp_server_s::run() {
+    DeviceScope scope(device);
+    std::unique_lock<std::mutex> lk(mu);
+    std::deque<int64_t> waiting;  // builds held back for memory, oldest first
+    for (;;) {
+        bool progressed = false;
+        // builds that waited for memory go first, in arrival order
+        std::deque<int64_t> turn;
+        turn.swap(waiting);
+        while (!queue.empty()) {
+            turn.push_back(queue.front());
+            queue.pop_front();
+        }
+        for (size_t i = 0; i < turn.size(); ++i) {
+            const int64_t id = turn[i];
+            Request& r = reqs[id];
+            if (static_cast<int>(inflight.size()) >= max_inflight) {
+                waiting.insert(waiting.end(), turn.begin() + static_cast<std::ptrdiff_t>(i), turn.end());
+                break;
+            }
+            if (r.kind == INTP_REQ_BUILD) {
+                if (admit_build(id, r))
+                    progressed = true;
+                else
+                    waiting.push_back(id);
+            } else {
+                launch_eval(id, r);
+                progressed = true;
+            }
+        }
+        const size_t before = inflight.size();
+        poll_inflight();
+        progressed = progressed || inflight.size() != before;
+        if (stop && queue.empty() && waiting.empty() && inflight.empty()) break;
+        if (!progressed) {
+            if (inflight.empty() && queue.empty() && waiting.empty())
+                wake.wait(lk, [&] { return stop || !queue.empty(); });
+            else
+                wake.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
+extern "C" {
+
+int intp_server_create(intp_ctx ctx, const intp_server_config* cfg, intp_server* out) {
+    if (!ctx || !out) return fail(INTP_E_INVALID, "null argument");
+    intp_server_config c{};
+    if (cfg) c = *cfg;
+    if (c.streams == 0) c.streams = 4;
+    if (c.max_inflight == 0) c.max_inflight = 256;
+    if (c.streams < 1 || c.streams > kMaxServeStreams || c.max_inflight < 1 || c.hbm_budget < 0)
+        return fail(INTP_E_INVALID, "server config out of range");
+    std::unique_ptr<intp_server_s> s(new intp_server_s());
+    s->ctx = ctx;
+    s->device = ctx->device;
+    DeviceScope scope(s->device);
+    if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
+    if (c.hbm_budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        c.hbm_budget = std::max<int64_t>(int64_t{1} << 30, static_cast<int64_t>(free_b) - (int64_t{8} << 30));
+    }
+    s->budget = c.hbm_budget;
+    s->max_inflight = c.max_inflight;
+    for (int i = 0; i < c.streams; ++i) {
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        s->streams.push_back(st);
+    }
+    intp_server_s* raw = s.get();
+    s->dispatcher = std::thread([raw] { raw->run(); });
+    *out = s.release();
+    return INTP_OK;
+}
+
+int intp_server_destroy(intp_server s) {
+    if (!s) return fail(INTP_E_INVALID, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->wake.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    DeviceScope scope(s->device);
+    for (auto& kv : s->splines) {
+        (void)intp_spline_release(kv.second.h);
+        if (kv.second.built) (void)hipEventDestroy(kv.second.built);
+    }
+    for (auto& kv : s->reqs)
+        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+    for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
+    for (hipStream_t st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    delete s;
+    return INTP_OK;
+}
+
+int intp_serve_build(intp_server s, const intp_grid_desc* desc, const double* values, int32_t precision,
+                     int32_t sweeps, int64_t* request) {
+    if (!s || !desc || !values || !request) return fail(INTP_E_INVALID, "null argument");
+    if (precision != INTP_PREC_F64 && precision != INTP_PREC_F32_REFINED)
+        return fail(INTP_E_INVALID, "precision must be INTP_PREC_F64 or INTP_PREC_F32_REFINED");
+    if (desc->dim < 1 || desc->dim > INTP_MAX_DIM) return fail(INTP_E_UNSUPPORTED, "dimension without kernels");
+    for (int d = 0; d < desc->dim; ++d)
+        if (desc->n[d] < 1) return fail(INTP_E_INVALID, "empty dimension");
+    Request r;
+    r.kind = INTP_REQ_BUILD;
+    r.desc = *desc;
+    for (int d = 0; d < desc->dim; ++d)
+        if (desc->coords[d]) {
+            r.coords_copy.emplace_back(desc->coords[d], desc->coords[d] + desc->n[d] + (desc->periodic[d] ? 1 : 0));
+            r.desc.coords[d] = r.coords_copy.back().data();
+        }
+    r.values = values;
+    r.precision = precision;
+    r.sweeps = sweeps;
+    r.footprint = build_footprint(*desc, precision);
+    r.bytes = build_bytes(*desc, precision, sweeps);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        const int64_t id = s->next_id++;
+        r.spline = id;
+        r.t_submit = s->now();
+        if (s->first_submit < 0) s->first_submit = r.t_submit;
+        s->reqs.emplace(id, std::move(r));
+        s->queue.push_back(id);
+        ++s->unfinished;
+        *request = id;
+    }
+    s->wake.notify_one();
+    return INTP_OK;
+}
+
+int intp_serve_eval(intp_server s, int64_t spline, int64_t npts, const double* coords, int64_t cs_dim,
+                    int64_t cs_pt, int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
+                    uint32_t flags, int64_t* request) {
+    if (!s || !request || (npts > 0 && (!coords || !out)) || nout < 1 || !orders)
+        return fail(INTP_E_INVALID, "null argument");
+    if (flags & (INTP_EVAL_BOUNDS_CHECK | INTP_EVAL_HOST_MEM | INTP_EVAL_TIMED))
+        return fail(INTP_E_INVALID, "served evaluations take device buffers without bounds checks or timing");
+    Request r;
+    r.kind = INTP_REQ_EVAL;
+    r.spline = spline;
+    r.npts = npts;
+    r.coords = coords;
+    r.cs_dim = cs_dim;
+    r.cs_pt = cs_pt;
+    r.nout = nout;
+    r.out = out;
+    r.os_out = os_out;
+    r.os_pt = os_pt;
+    r.flags = flags;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto b = s->reqs.find(spline);
+        if (b == s->reqs.end() && spline > 0 && spline < s->next_id)
+            return fail(INTP_E_EVICTED, "spline " + std::to_string(spline) + " is not resident");
+        if (b == s->reqs.end() || b->second.kind != INTP_REQ_BUILD)
+            return fail(INTP_E_INVALID, "no build request " + std::to_string(spline));
+        const intp_grid_desc& d = b->second.desc;
+        r.orders.assign(orders, orders + static_cast<int64_t>(nout) * d.dim);
+        int64_t n = 1;
+        for (int i = 0; i < d.dim; ++i) n *= d.n[i];
+        const double s_out = (flags & INTP_EVAL_OUT_F32) ? 4 : 8;
+        const double s_tab = b->second.precision == INTP_PREC_F32_REFINED ? 4 : 8;
+        r.bytes = static_cast<double>(npts) * (d.dim * 8 + nout * s_out) + static_cast<double>(n) * s_tab;
+        const int64_t id = s->next_id++;
+        r.t_submit = s->now();
+        if (s->first_submit < 0) s->first_submit = r.t_submit;
+        s->reqs.emplace(id, std::move(r));
+        s->queue.push_back(id);
+        ++s->unfinished;
+        *request = id;
+    }
+    s->wake.notify_one();
+    return INTP_OK;
+}
+
+int intp_serve_poll(intp_server s, int64_t request, int32_t* state, int32_t* status) {
+    if (!s || !state) return fail(INTP_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(request);
+    if (it == s->reqs.end()) return fail(INTP_E_INVALID, "no such request");
+    *state = it->second.state;
+    if (status) *status = it->second.status;
+    return INTP_OK;
+}
+
+int intp_serve_wait(intp_server s, int64_t request) {
+    if (!s) return fail(INTP_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->reqs.find(request) == s->reqs.end()) return fail(INTP_E_INVALID, "no such request");
+    int st = INTP_OK;
+    // looked up on every wake-up: the table may grow (or a statistics reset
+    // may forget the finished request) while this thread sleeps
+    s->done.wait(lk, [&] {
+        auto it = s->reqs.find(request);
+        if (it == s->reqs.end()) return true;
+        st = it->second.status;
+        return it->second.state >= INTP_REQ_DONE;
+    });
+    if (st != INTP_OK) return fail(st, "request " + std::to_string(request) + " failed: " + intp_status_string(st));
+    return INTP_OK;
+}
+
+int intp_serve_drain(intp_server s) {
+    if (!s) return fail(INTP_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->done.wait(lk, [&] { return s->unfinished == 0; });
+    return INTP_OK;
+}
+
+int intp_serve_release(intp_server s, int64_t spline) {
+    if (!s) return fail(INTP_E_INVALID, "null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->splines.find(spline);
+        if (it == s->splines.end()) return INTP_OK;  // already evicted
+        it->second.released = true;
+    }
+    s->wake.notify_one();
+    return INTP_OK;
+}
+
+int intp_serve_spline(intp_server s, int64_t spline, intp_spline* out) {
+    if (!s || !out) return fail(INTP_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->splines.find(spline);
+    if (it == s->splines.end() || it->second.released)
+        return fail(INTP_E_EVICTED, "spline " + std::to_string(spline) + " is not resident");
+    if (!it->second.ready) return fail(INTP_E_INVALID, "spline " + std::to_string(spline) + " is still building");
+    const int rc = intp_spline_retain(it->second.h);
+    if (rc != INTP_OK) return rc;
+    *out = it->second.h;
+    return INTP_OK;
+}
+
+int intp_serve_stats(intp_server s, int32_t reset, intp_serve_counters* out) {
+    if (!s || !out) return fail(INTP_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    for (int k = 0; k < 2; ++k) {
+        intp_serve_type_counters& t = out->type[k];
+        std::vector<double> v = s->lat[k];
+        t.completed = static_cast<int64_t>(v.size());
+        t.failed = s->failed[k];
+        t.bytes = s->bytes[k];
+        if (!v.empty()) {
+            std::sort(v.begin(), v.end());
+            auto pct = [&](double p) {
+                const size_t i = std::min(v.size() - 1, static_cast<size_t>(p * static_cast<double>(v.size() - 1) + 0.5));
+                return v[i];
+            };
+            t.latency_p50_ms = pct(0.50);
+            t.latency_p99_ms = pct(0.99);
+            double sum = 0;
+            for (double x : v) sum += x;
+            t.latency_mean_ms = sum / static_cast<double>(v.size());
+            t.latency_max_ms = v.back();
+        }
+    }
+    out->resident_splines = static_cast<int64_t>(s->splines.size());
+    out->resident_bytes = s->resident;
+    out->evictions = s->evictions;
+    out->admission_waits = s->admission_waits;
+    out->first_submit_s = s->first_submit;
+    out->last_done_s = s->last_done;
+    if (reset) {
+        for (int k = 0; k < 2; ++k) {
+            s->lat[k].clear();
+            s->bytes[k] = 0;
+            s->failed[k] = 0;
+        }
+        s->evictions = s->admission_waits = 0;
+        s->first_submit = -1;
+        s->last_done = 0;
+        // forget finished requests, keeping the build records of resident
+        // splines (evaluations look their grids up)
+        for (auto it = s->reqs.begin(); it != s->reqs.end();) {
+            const Request& r = it->second;
+            const bool finished = r.state >= INTP_REQ_DONE;
+            const bool keep = r.kind == INTP_REQ_BUILD && s->splines.count(it->first);
+            if (finished && !keep)
+                it = s->reqs.erase(it);
+            else
+                ++it;
+        }
+    }
+    return INTP_OK;
+}
+
+}  // extern "C"

@@ -40,7 +40,9 @@ typedef enum {
                                std::domain_error (Interpolation.hpp:482-491) */
     INTP_E_NOMEM = 3,       /* device allocation failed; std::bad_alloc  */
     INTP_E_HIP = 4,         /* HIP runtime error                         */
-    INTP_E_UNSUPPORTED = 5  /* dimension/order without compiled kernels  */
+    INTP_E_UNSUPPORTED = 5, /* dimension/order without compiled kernels  */
+    INTP_E_EVICTED = 6      /* serving: the spline is no longer resident
+                               (evicted, released or its build failed)    */
 } intp_status;
 
 /* where a caller's buffer lives */
@@ -213,6 +215,97 @@ int intp_batch_run(intp_batch b, const double* values, const double* coords, con
 /* The tile shapes in use and the autotuner's timings, as text. */
 int intp_batch_tuning(intp_batch b, char* buf, int64_t capacity);
 int intp_batch_download_ctrl(intp_batch b, int32_t spline, double* host, int64_t count);
+
+/* ---- serving: a request loop over resident splines ------------------ */
+/* No reference counterpart (the reference builds and evaluates in the
+ * caller's thread; InterpolationFunctionTemplate reuse,
+ * InterpolationTemplate.hpp:118-143, is the closest analogue).  A server
+ * owns the splines it builds.  A dispatcher thread admits every queued
+ * request on each turn of its loop while earlier ones are still running
+ * (continuous batching), spreads them over several HIP streams, and
+ * records completion with HIP events:
+ *   build: admitted when its control points fit the HBM budget next to
+ *     the resident splines; least-recently-used splines with no pending
+ *     evaluation are evicted to make room; a build that fits only after
+ *     running work completes waits in the queue (evaluations pass it);
+ *   eval: runs on the spline's table once its build has finished (stream
+ *     ordered, so an eval may be submitted right after its build).
+ * Results are those of intp_build / intp_build_refined / intp_eval on the
+ * same inputs, whatever else runs beside them.  Buffers passed to a
+ * request (values, coords, out: device memory) must stay valid until it
+ * completes.                                                             */
+typedef struct intp_server_s* intp_server;
+
+typedef struct {
+    int64_t hbm_budget;   /* bytes for resident control points; 0: free HBM
+                             at creation minus 8 GiB (at least 1 GiB)      */
+    int32_t streams;      /* HIP streams the requests are spread over
+                             (1..16; 0: 4)                                  */
+    int32_t max_inflight; /* launched, unfinished requests (0: 256)        */
+} intp_server_config;
+
+#define INTP_REQ_BUILD 0
+#define INTP_REQ_EVAL 1
+
+/* request states */
+#define INTP_REQ_QUEUED 0
+#define INTP_REQ_RUNNING 1
+#define INTP_REQ_DONE 2
+#define INTP_REQ_FAILED 3
+
+/* build precision */
+#define INTP_PREC_F64 0
+#define INTP_PREC_F32_REFINED 1
+
+typedef struct {
+    int64_t completed;
+    int64_t failed;
+    double bytes;           /* algorithmic bytes of the completed requests */
+    double latency_p50_ms;  /* submit to completion */
+    double latency_p99_ms;
+    double latency_mean_ms;
+    double latency_max_ms;
+} intp_serve_type_counters;
+
+typedef struct {
+    intp_serve_type_counters type[2]; /* [INTP_REQ_BUILD], [INTP_REQ_EVAL] */
+    int64_t resident_splines;
+    int64_t resident_bytes;
+    int64_t evictions;
+    int64_t admission_waits;       /* turns a build waited for memory      */
+    double first_submit_s;         /* server clock (s) of the first submit */
+    double last_done_s;            /* and of the last completion, since the
+                                      last reset                           */
+} intp_serve_counters;
+
+int intp_server_create(intp_ctx ctx, const intp_server_config* cfg, intp_server* out);
+/* Waits for every submitted request, then frees the server's splines. */
+int intp_server_destroy(intp_server s);
+/* Queues a build of `values` (device, C-order) on grid `desc`; precision
+ * INTP_PREC_F64 or INTP_PREC_F32_REFINED (with `sweeps` refinement
+ * sweeps).  *request receives the request id, which is also the id of the
+ * spline it builds.                                                      */
+int intp_serve_build(intp_server s, const intp_grid_desc* desc, const double* values, int32_t precision,
+                     int32_t sweeps, int64_t* request);
+/* Queues an evaluation of spline `spline` (a build request id) with
+ * intp_eval's arguments (device coords and out; orders is copied).       */
+int intp_serve_eval(intp_server s, int64_t spline, int64_t npts, const double* coords, int64_t cs_dim,
+                    int64_t cs_pt, int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
+                    uint32_t flags, int64_t* request);
+/* State (INTP_REQ_*) and, once finished, the intp_status of a request. */
+int intp_serve_poll(intp_server s, int64_t request, int32_t* state, int32_t* status);
+/* Blocks until the request is done or failed; returns its status.       */
+int intp_serve_wait(intp_server s, int64_t request);
+/* Blocks until every request submitted so far has finished.             */
+int intp_serve_drain(intp_server s);
+/* The client needs the spline no more: it is freed once idle.           */
+int intp_serve_release(intp_server s, int64_t spline);
+/* A new reference to a resident, built spline (release it with
+ * intp_spline_release).                                                  */
+int intp_serve_spline(intp_server s, int64_t spline, intp_spline* out);
+/* Counters and latency percentiles of the requests finished since the
+ * last reset (reset != 0 starts a new period after reading).             */
+int intp_serve_stats(intp_server s, int32_t reset, intp_serve_counters* out);
 
 /* Durations (ms) of the calls made with INTP_EVAL_TIMED on this thread
  * since the last query, oldest first: group_ms[i] covers the grouping

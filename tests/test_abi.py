@@ -40,9 +40,25 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_status_strings(lib):
-    for code in range(6):
-        assert lib.intp_status_string(code)
+    for code in range(7):
+        assert lib.intp_status_string(code) != b"unknown status"
+    assert lib.intp_status_string(99) == b"unknown status"
     assert lib.intp_last_error() is not None
+
+
+def test_serving_entry_points_reject_null_arguments_without_a_device(lib):
+    from bsplineinterpolation_amd import _lib
+    h = C.c_void_p()
+    rid = C.c_int64()
+    assert lib.intp_server_create(None, None, C.byref(h)) == _lib.INTP_E_INVALID
+    assert lib.intp_server_destroy(None) == _lib.INTP_E_INVALID
+    assert lib.intp_serve_build(None, None, None, 0, 0, C.byref(rid)) == _lib.INTP_E_INVALID
+    assert lib.intp_serve_eval(None, 1, 1, None, 1, 1, 1, None, None, 1, 1, 0, C.byref(rid)) == _lib.INTP_E_INVALID
+    assert lib.intp_serve_wait(None, 1) == _lib.INTP_E_INVALID
+    assert lib.intp_serve_drain(None) == _lib.INTP_E_INVALID
+    assert lib.intp_serve_stats(None, 0, None) == _lib.INTP_E_INVALID
+    st = C.c_int32()
+    assert lib.intp_serve_poll(None, 1, C.byref(st), None) == _lib.INTP_E_INVALID
 
 
 def test_null_arguments_are_rejected_without_a_device(lib):
