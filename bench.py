@@ -147,6 +147,9 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
             "periodic_1d_order6_n8192": {"error_vs_f64": err1, "note": "periodic order >= 5: FP64 factors"}}
 
 
+SERVE_RATES = (1e2, 3e2, 1e3, 3e3, 1e4)
+
+
 def measure_batched(amd, torch, steps: int):
     """The batched form: 4096 mixed 1D/2D/3D FP64 splines (orders 1..5,
     ~25% periodic dimensions), built and evaluated at 256 points each per
@@ -196,6 +199,159 @@ def measure_batched(amd, torch, steps: int):
             "parity": "control points and values bit-identical to the single-spline path (tests/test_gpu_batch.py)"}
 
 
+def measure_serving(amd, torch, seconds: float, rates, burst: int, batch_gbs=None):
+    """The serving workload: for each arrival rate, Poisson requests for
+    ``seconds`` (30% builds of 2D/3D grids of 32..1024 / 32..192 per side,
+    orders 3..5, FP64 or refined FP32; 70% evaluations of 1e3..1e6 points,
+    half with first partials) submitted from this thread at their arrival
+    times to one server (16 GiB HBM budget, 4 streams); per rate:
+    requests/s, algorithmic bytes/s and p50/p99 latency per type.  Then a
+    burst of ``burst`` requests submitted at once (saturation), and the same
+    requests run directly back to back on one stream (the offline batch of
+    the same work): serving / offline throughput is the batching ratio.
+    Submissions use the C entry points with arguments prepared beforehand,
+    so that the client thread keeps up with 1e4 requests/s."""
+    import ctypes as C
+    from bsplineinterpolation_amd import _lib, datagen
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    max_n, max_q = 192 ** 3, 1_000_000
+    g = torch.Generator(device="cpu").manual_seed(7)
+    pool_v = torch.randn(max_n, generator=g, dtype=torch.float64).to(dev)
+    pool_q = torch.rand((max_q, 3), generator=g, dtype=torch.float64).to(dev)
+    nbuf = 256
+    outs = torch.empty((nbuf, 4 * max_q), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    dp = C.POINTER(C.c_double)
+    vptr, qptr = C.cast(pool_v.data_ptr(), dp), C.cast(pool_q.data_ptr(), dp)
+    optr = [C.cast(outs[i].data_ptr(), dp) for i in range(nbuf)]
+
+    def prepare(reqs):
+        args = []
+        for i, rq in enumerate(reqs):
+            if rq["kind"] == "build":
+                D = len(rq["shape"])
+                desc = _lib.GridDesc()
+                desc.dim, desc.order = D, rq["order"]
+                for d in range(D):
+                    desc.n[d], desc.xmin[d], desc.xmax[d] = rq["shape"][d], 0.0, 1.0
+                prec = _lib.INTP_PREC_F64 if rq["precision"] == "f64" else _lib.INTP_PREC_F32_REFINED
+                args.append((desc, prec))
+            else:
+                D = len(reqs[rq["target"]]["shape"])
+                ords = [0] * D + (sum(([int(k == d) for k in range(D)] for d in range(D)), []) if rq["derivs"] else [])
+                nout = len(ords) // D
+                args.append(((C.c_int32 * len(ords))(*ords), nout, optr[i % nbuf]))
+        return args
+
+    rid = C.c_int64()
+
+    def submit(srv, reqs, args, ids, i):
+        rq = reqs[i]
+        if rq["kind"] == "build":
+            desc, prec = args[i]
+            _lib.check(lib.intp_serve_build(srv.h, C.byref(desc), vptr, prec, 1, C.byref(rid)))
+        else:
+            ords, nout, out = args[i]
+            rc = lib.intp_serve_eval(srv.h, ids[rq["target"]], rq["npts"], qptr, 1, 3, nout, ords, out,
+                                     rq["npts"], 1, 0, C.byref(rid))
+            if rc == _lib.INTP_E_EVICTED:  # target evicted before this submit
+                ids[i] = None
+                return
+            _lib.check(rc)
+        ids[i] = rid.value
+
+    def summarize(st, wall):
+        res = {}
+        for k in ("build", "eval"):
+            t = st[k]
+            res[k] = dict(completed=t["completed"], failed=t["failed"], per_s=t["completed"] / wall,
+                          gbs=t["bytes"] / wall / 1e9, p50_ms=t["p50_ms"], p99_ms=t["p99_ms"], max_ms=t["max_ms"])
+        res["gbs"] = (st["build"]["bytes"] + st["eval"]["bytes"]) / wall / 1e9
+        res.update(evictions=st["evictions"], admission_waits=st["admission_waits"], wall_s=wall)
+        return res
+
+    srv = amd.Server(hbm_budget=16 << 30, streams=4, max_inflight=256)
+    phases = []
+    for ph, rate in enumerate(rates):
+        reqs = datagen.workload_serving(rate=rate, seconds=seconds, phase=ph + 1)
+        args = prepare(reqs)
+        ids = [None] * len(reqs)
+        t0 = time.perf_counter()
+        lag = 0.0
+        for i, rq in enumerate(reqs):
+            while time.perf_counter() - t0 < rq["t"]:
+                pass
+            lag = max(lag, time.perf_counter() - t0 - rq["t"])
+            submit(srv, reqs, args, ids, i)
+        srv.drain()
+        st = srv.stats(reset=True)
+        res = summarize(st, st["last_done_s"] - st["first_submit_s"])
+        res.update(rate=rate, requests=len(reqs), max_submit_lag_ms=lag * 1e3)
+        phases.append(res)
+        print("bench: serving rate %g/s: %d requests, build p50/p99 %.2f/%.2f ms, eval p50/p99 %.2f/%.2f ms, "
+              "submit lag <= %.2f ms" % (rate, len(reqs), res["build"]["p50_ms"], res["build"]["p99_ms"],
+                                         res["eval"]["p50_ms"], res["eval"]["p99_ms"], lag * 1e3),
+              file=sys.stderr, flush=True)
+    # saturation: every request of the burst queued at once
+    reqs = datagen.workload_serving(rate=None, count=burst, phase=len(rates) + 1)
+    args = prepare(reqs)
+    ids = [None] * len(reqs)
+    t0 = time.perf_counter()
+    for i in range(len(reqs)):
+        submit(srv, reqs, args, ids, i)
+    submit_s = time.perf_counter() - t0
+    srv.drain()
+    st = srv.stats(reset=True)
+    sat = summarize(st, st["last_done_s"] - st["first_submit_s"])
+    sat["submit_s"] = submit_s
+    srv.close()
+    # the same requests as an offline batch: direct calls on one stream
+    stream = torch.cuda.current_stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    ctx = amd.Context.default()
+    hs = [None] * len(reqs)
+    nbytes = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, rq in enumerate(reqs):
+        if rq["kind"] == "build":
+            desc, prec = args[i]
+            h = C.c_void_p()
+            if prec == _lib.INTP_PREC_F64:
+                _lib.check(lib.intp_build(ctx.h, C.byref(desc), vptr, _lib.INTP_MEM_DEVICE, C.byref(h), sh))
+            else:
+                _lib.check(lib.intp_build_refined(ctx.h, C.byref(desc), vptr, _lib.INTP_MEM_DEVICE, 1, C.byref(h), sh))
+            hs[i] = h
+        else:
+            ords, nout, out = args[i]
+            _lib.check(lib.intp_eval(hs[rq["target"]], rq["npts"], qptr, 1, 3, nout, ords, out, rq["npts"], 1, 0, sh))
+    torch.cuda.synchronize()
+    off_s = time.perf_counter() - t0
+    for h in hs:
+        if h is not None:
+            lib.intp_spline_release(h)
+    for rq in reqs:
+        if rq["kind"] == "build":
+            D, N = len(rq["shape"]), int(np.prod(rq["shape"]))
+            nbytes += (2 * D * N * 8 if rq["precision"] == "f64" else
+                       2 * 2 * D * N * 4 + (2 * D * N * 8 + 8 * N) + N * 4)
+        else:
+            b = reqs[rq["target"]]
+            D, N = len(b["shape"]), int(np.prod(b["shape"]))
+            nbytes += rq["npts"] * (D * 8 + (1 + (D if rq["derivs"] else 0)) * 8) + N * (
+                8 if b["precision"] == "f64" else 4)
+    offline = dict(wall_s=off_s, gbs=nbytes / off_s / 1e9)
+    out = dict(workload="Poisson arrivals, %g s per rate; 30%% builds (2D sides 32..1024 or 3D sides 32..192, "
+                        "orders 3..5, FP64 or refined FP32 with 1 sweep), 70%% evaluations of 1e3..1e6 points "
+                        "(half with first partials) on the 16 latest builds; 16 GiB HBM budget, 4 streams" % seconds,
+               rates=phases, saturation=dict(requests=burst, **sat), offline_same_requests=offline,
+               ratio_vs_offline=sat["gbs"] / offline["gbs"])
+    if batch_gbs:
+        out["ratio_vs_batched_form_gbs"] = sat["gbs"] / batch_gbs
+    return out
+
+
 def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
     """Reference CPU path, bounded sample.  Returns (dict, error stats)."""
     import oracle
@@ -238,6 +394,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the refined-FP32 form measurement")
     ap.add_argument("--only-forms", action="store_true",
                     help="measure only the refined-FP32 form (profiling runs); prints its object")
+    ap.add_argument("--only-serving", action="store_true", help="measure only the serving loop; prints its object")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of arrivals per serving rate")
+    ap.add_argument("--serve-burst", type=int, default=1500, help="requests of the saturation burst")
     ap.add_argument("--only-batched", action="store_true",
                     help="measure only the batched form (4096 mixed splines); prints its object")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -255,6 +414,9 @@ def main():
     n, nq = args.n, args.queries
     if args.only_batched:
         print(json.dumps(measure_batched(amd, torch, args.steps)))
+        return
+    if args.only_serving:
+        print(json.dumps(measure_serving(amd, torch, args.serve_seconds, SERVE_RATES, args.serve_burst)))
         return
     if args.only_forms:
         forms = measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)
@@ -368,6 +530,9 @@ def main():
         result["forms"] = measure_refined_form(amd, torch, max(3, args.steps // 2), 2, n, nq)
         print("bench: batched form (4096 mixed splines)", file=sys.stderr, flush=True)
         result["forms"]["batched"] = measure_batched(amd, torch, max(5, args.steps))
+        print("bench: serving loop", file=sys.stderr, flush=True)
+        result["forms"]["serving"] = measure_serving(amd, torch, args.serve_seconds, SERVE_RATES, args.serve_burst,
+                                                     result["forms"]["batched"]["gbs"])
 
     if not args.no_cpu:
         m = min(args.cpu_sample, nq)

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <deque>
 #include <list>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -117,36 +118,43 @@ struct intp_server_s {
     size_t next_stream = 0;
     Clock::time_point t0 = Clock::now();
 
+    // Guarded by mu: the tables below and every Request's state, status,
+    // t_done and done event.  A Request's arguments are written before it
+    // is queued and only read afterwards, so the dispatcher reads them
+    // without the lock; the lock is not held across HIP launches, so
+    // submissions never wait for one.
     std::mutex mu;
     std::condition_variable wake;  // dispatcher: new work
     std::condition_variable done;  // clients: a request finished
     bool stop = false;
     int64_t next_id = 1;
     std::deque<int64_t> queue;
-    std::unordered_map<int64_t, Request> reqs;
+    std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     std::unordered_map<int64_t, Resident> splines;
     std::list<int64_t> lru;  // front: least recently used
     int64_t resident = 0;
     std::vector<int64_t> inflight;
     int64_t unfinished = 0;  // submitted, not yet done or failed
-    std::vector<hipEvent_t> spare_events;
 
-    // statistics of the current period
+    // statistics of the current period (mu)
     std::vector<double> lat[2];
     double bytes[2] = {0, 0};
     int64_t failed[2] = {0, 0};
     int64_t evictions = 0, admission_waits = 0;
     double first_submit = -1, last_done = 0;
 
+    // dispatcher thread only
+    std::vector<hipEvent_t> spare_events;
+    std::vector<intp_spline> to_release;
+
     std::thread dispatcher;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     void run();
-    void finish(int64_t id, Request& r, int status);
-    bool admit_build(int64_t id, Request& r);
-    bool launch_eval(int64_t id, Request& r);
+    void finish(Request& r, int status);
+    int dispatch(int64_t id, std::unique_lock<std::mutex>& lk);
     void poll_inflight();
-    void evict_idle_released();
+    void drop(std::unordered_map<int64_t, Resident>::iterator it);
     hipEvent_t take_event();
 };
 
@@ -161,8 +169,8 @@ hipEvent_t intp_server_s::take_event() {
     return e;
 }
 
-// Marks a request finished (caller holds mu).
-void intp_server_s::finish(int64_t id, Request& r, int status) {
+// Marks a request finished (dispatcher, holding mu).
+void intp_server_s::finish(Request& r, int status) {
     r.status = status;
     r.state = status == INTP_OK ? INTP_REQ_DONE : INTP_REQ_FAILED;
     r.t_done = now();
@@ -177,140 +185,150 @@ void intp_server_s::finish(int64_t id, Request& r, int status) {
         spare_events.push_back(r.done);
         r.done = nullptr;
     }
-    (void)id;
     --unfinished;
     done.notify_all();
 }
 
-// Frees released splines that nothing uses any more (caller holds mu).
-void intp_server_s::evict_idle_released() {
-    for (auto it = splines.begin(); it != splines.end();) {
-        Resident& s = it->second;
-        if (s.released && s.ready && s.pins == 0) {
-            resident -= s.bytes;
-            lru.erase(s.lru);
-            if (s.built) spare_events.push_back(s.built);
-            (void)intp_spline_release(s.h);
-            it = splines.erase(it);
-        } else {
-            ++it;
-        }
-    }
-}
-
-// Admits and launches a build if it fits the budget (caller holds mu).
-bool intp_server_s::admit_build(int64_t id, Request& r) {
-    if (r.footprint > budget) {
-        finish(id, r, fail(INTP_E_NOMEM, "build larger than the server's HBM budget"));
-        return true;
-    }
-    // evict least-recently-used idle splines until it fits
-    for (auto it = lru.begin(); resident + r.footprint > budget && it != lru.end();) {
-        const int64_t victim = *it;
-        Resident& s = splines[victim];
-        if (s.ready && s.pins == 0) {
-            resident -= s.bytes;
-            if (s.built) spare_events.push_back(s.built);
-            (void)intp_spline_release(s.h);
-            it = lru.erase(it);
-            splines.erase(victim);
-            ++evictions;
-        } else {
-            ++it;
-        }
-    }
-    if (resident + r.footprint > budget) {
-        if (!r.waited) ++admission_waits;
-        r.waited = true;
-        return false;  // wait for running work to finish
-    }
-    hipStream_t st = streams[next_stream++ % streams.size()];
-    intp_spline h = nullptr;
-    const int rc = r.precision == INTP_PREC_F32_REFINED
-                       ? intp_build_refined(ctx, &r.desc, r.values, INTP_MEM_DEVICE, r.sweeps, &h, st)
-                       : intp_build(ctx, &r.desc, r.values, INTP_MEM_DEVICE, &h, st);
-    if (rc != INTP_OK) {
-        finish(id, r, rc);
-        return true;
-    }
-    hipEvent_t ev = take_event();
-    hipEvent_t built = take_event();
-    if (!ev || !built || hipEventRecord(ev, st) != hipSuccess || hipEventRecord(built, st) != hipSuccess) {
-        (void)intp_spline_release(h);
-        if (ev) spare_events.push_back(ev);
-        if (built) spare_events.push_back(built);
-        finish(id, r, fail(INTP_E_HIP, "hipEventRecord"));
-        return true;
-    }
-    Resident s;
-    s.h = h;
-    s.bytes = resident_bytes_of(r.desc, r.precision);
-    s.built = built;
-    lru.push_back(id);
-    s.lru = std::prev(lru.end());
-    splines.emplace(id, s);
-    resident += s.bytes;
-    r.done = ev;
-    r.state = INTP_REQ_RUNNING;
-    inflight.push_back(id);
-    return true;
-}
-
-// Launches an evaluation (caller holds mu).
-bool intp_server_s::launch_eval(int64_t id, Request& r) {
-    auto it = splines.find(r.spline);
-    if (it == splines.end() || it->second.released) {
-        finish(id, r, fail(INTP_E_EVICTED, "spline " + std::to_string(r.spline) + " is not resident"));
-        return true;
-    }
+// Removes a resident spline; its handle is released after the lock drops
+// (dispatcher, holding mu).
+void intp_server_s::drop(std::unordered_map<int64_t, Resident>::iterator it) {
     Resident& s = it->second;
-    hipStream_t st = streams[next_stream++ % streams.size()];
-    if (!s.ready && hipStreamWaitEvent(st, s.built, 0) != hipSuccess) {
-        finish(id, r, fail(INTP_E_HIP, "hipStreamWaitEvent"));
-        return true;
-    }
-    const int rc = intp_eval(s.h, r.npts, r.coords, r.cs_dim, r.cs_pt, r.nout, r.orders.data(), r.out, r.os_out,
-                             r.os_pt, r.flags, st);
-    if (rc != INTP_OK) {
-        finish(id, r, rc);
-        return true;
-    }
-    hipEvent_t ev = take_event();
-    if (!ev || hipEventRecord(ev, st) != hipSuccess) {
-        if (ev) spare_events.push_back(ev);
-        (void)hipStreamSynchronize(st);  // the launch is in flight; finish it before reporting
-        finish(id, r, fail(INTP_E_HIP, "hipEventRecord"));
-        return true;
-    }
-    ++s.pins;
-    lru.splice(lru.end(), lru, s.lru);  // most recently used
-    r.done = ev;
-    r.state = INTP_REQ_RUNNING;
-    inflight.push_back(id);
-    return true;
+    resident -= s.bytes;
+    lru.erase(s.lru);
+    if (s.built) spare_events.push_back(s.built);
+    if (s.h) to_release.push_back(s.h);
+    splines.erase(it);
 }
 
-// Completes launched requests whose events have fired (caller holds mu).
+// Admits and launches one request.  Returns 1 if it ran or failed, 0 if a
+// build must wait for memory, -1 if the in-flight limit is reached.
+// Called and returns with lk held; drops it around the HIP launches.
+int intp_server_s::dispatch(int64_t id, std::unique_lock<std::mutex>& lk) {
+    if (static_cast<int>(inflight.size()) >= max_inflight) return -1;
+    Request& r = *reqs[id];
+    hipStream_t st = streams[next_stream++ % streams.size()];
+    if (r.kind == INTP_REQ_BUILD) {
+        if (r.footprint > budget) {
+            finish(r, fail(INTP_E_NOMEM, "build larger than the server's HBM budget"));
+            return 1;
+        }
+        // evict least-recently-used idle splines until it fits
+        for (auto it = lru.begin(); resident + r.footprint > budget && it != lru.end();) {
+            auto sp = splines.find(*it++);
+            if (sp->second.ready && sp->second.pins == 0) {
+                drop(sp);
+                ++evictions;
+            }
+        }
+        if (resident + r.footprint > budget) {
+            if (!r.waited) ++admission_waits;
+            r.waited = true;
+            return 0;  // wait for running work to finish
+        }
+        // reserve the memory under the request's id, then launch unlocked
+        Resident res;
+        res.bytes = resident_bytes_of(r.desc, r.precision);
+        lru.push_back(id);
+        res.lru = std::prev(lru.end());
+        resident += res.bytes;
+        splines.emplace(id, res);
+        lk.unlock();
+        for (intp_spline h : to_release) (void)intp_spline_release(h);
+        to_release.clear();
+        intp_spline h = nullptr;
+        int rc = r.precision == INTP_PREC_F32_REFINED
+                     ? intp_build_refined(ctx, &r.desc, r.values, INTP_MEM_DEVICE, r.sweeps, &h, st)
+                     : intp_build(ctx, &r.desc, r.values, INTP_MEM_DEVICE, &h, st);
+        hipEvent_t ev = nullptr, built = nullptr;
+        if (rc == INTP_OK) {
+            ev = take_event();
+            built = take_event();
+            if (!ev || !built || hipEventRecord(ev, st) != hipSuccess || hipEventRecord(built, st) != hipSuccess) {
+                (void)hipStreamSynchronize(st);  // the build is in flight; let it finish before dropping it
+                rc = fail(INTP_E_HIP, "hipEventRecord");
+            }
+        }
+        lk.lock();
+        auto it = splines.find(id);
+        if (rc != INTP_OK) {
+            if (ev) spare_events.push_back(ev);
+            if (built) spare_events.push_back(built);
+            it->second.h = h;  // released with the entry
+            drop(it);
+            finish(r, rc);
+            return 1;
+        }
+        it->second.h = h;
+        it->second.built = built;
+        r.done = ev;
+    } else {
+        auto it = splines.find(r.spline);
+        if (it == splines.end() || it->second.released) {
+            finish(r, fail(INTP_E_EVICTED, "spline " + std::to_string(r.spline) + " is not resident"));
+            return 1;
+        }
+        Resident& s = it->second;
+        ++s.pins;  // not evictable while the launch is prepared and runs
+        lru.splice(lru.end(), lru, s.lru);
+        const intp_spline h = s.h;
+        const hipEvent_t wait_for = s.ready ? nullptr : s.built;
+        lk.unlock();
+        for (intp_spline v : to_release) (void)intp_spline_release(v);
+        to_release.clear();
+        int rc = INTP_OK;
+        if (wait_for && hipStreamWaitEvent(st, wait_for, 0) != hipSuccess) rc = fail(INTP_E_HIP, "hipStreamWaitEvent");
+        if (rc == INTP_OK)
+            rc = intp_eval(h, r.npts, r.coords, r.cs_dim, r.cs_pt, r.nout, r.orders.data(), r.out, r.os_out, r.os_pt,
+                           r.flags, st);
+        hipEvent_t ev = nullptr;
+        if (rc == INTP_OK) {
+            ev = take_event();
+            if (!ev || hipEventRecord(ev, st) != hipSuccess) {
+                (void)hipStreamSynchronize(st);  // the launch is in flight; finish it before reporting
+                if (ev) spare_events.push_back(ev);
+                ev = nullptr;
+                rc = fail(INTP_E_HIP, "hipEventRecord");
+            }
+        }
+        lk.lock();
+        if (rc != INTP_OK) {
+            --splines.find(r.spline)->second.pins;
+            finish(r, rc);
+            return 1;
+        }
+        r.done = ev;
+    }
+    r.state = INTP_REQ_RUNNING;
+    inflight.push_back(id);
+    return 1;
+}
+
+// Completes launched requests whose events have fired, and frees released
+// splines nothing uses any more (dispatcher, holding mu).
 void intp_server_s::poll_inflight() {
     size_t keep = 0;
     for (size_t i = 0; i < inflight.size(); ++i) {
         const int64_t id = inflight[i];
-        Request& r = reqs[id];
+        Request& r = *reqs[id];
         const hipError_t q = hipEventQuery(r.done);
         if (q == hipErrorNotReady) {
             inflight[keep++] = id;
             continue;
         }
         auto it = splines.find(r.spline);
-        if (r.kind == INTP_REQ_EVAL) {
-            if (it != splines.end()) --it->second.pins;
-        } else if (it != splines.end()) {
-            it->second.ready = true;
+        if (it != splines.end()) {
+            if (r.kind == INTP_REQ_EVAL)
+                --it->second.pins;
+            else
+                it->second.ready = true;
         }
-        finish(id, r, q == hipSuccess ? INTP_OK : hip_fail(q, "request"));
+        finish(r, q == hipSuccess ? INTP_OK : hip_fail(q, "request"));
     }
     inflight.resize(keep);
-    evict_idle_released();
+    for (auto it = splines.begin(); it != splines.end();) {
+        auto cur = it++;
+        if (cur->second.released && cur->second.ready && cur->second.pins == 0) drop(cur);
+    }
 }
 
 void intp_server_s::run() {
@@ -327,25 +345,25 @@ void intp_server_s::run() {
             queue.pop_front();
         }
         for (size_t i = 0; i < turn.size(); ++i) {
-            const int64_t id = turn[i];
-            Request& r = reqs[id];
-            if (static_cast<int>(inflight.size()) >= max_inflight) {
+            const int rc = dispatch(turn[i], lk);
+            if (rc < 0) {
                 waiting.insert(waiting.end(), turn.begin() + static_cast<std::ptrdiff_t>(i), turn.end());
                 break;
             }
-            if (r.kind == INTP_REQ_BUILD) {
-                if (admit_build(id, r))
-                    progressed = true;
-                else
-                    waiting.push_back(id);
-            } else {
-                launch_eval(id, r);
+            if (rc == 0)
+                waiting.push_back(turn[i]);
+            else
                 progressed = true;
-            }
         }
         const size_t before = inflight.size();
         poll_inflight();
         progressed = progressed || inflight.size() != before;
+        if (!to_release.empty()) {
+            lk.unlock();
+            for (intp_spline h : to_release) (void)intp_spline_release(h);
+            to_release.clear();
+            lk.lock();
+        }
         if (stop && queue.empty() && waiting.empty() && inflight.empty()) break;
         if (!progressed) {
             if (inflight.empty() && queue.empty() && waiting.empty())
@@ -403,7 +421,7 @@ int intp_server_destroy(intp_server s) {
         if (kv.second.built) (void)hipEventDestroy(kv.second.built);
     }
     for (auto& kv : s->reqs)
-        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+        if (kv.second->done) (void)hipEventDestroy(kv.second->done);
     for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
     for (hipStream_t st : s->streams) {
         (void)hipStreamSynchronize(st);
@@ -440,7 +458,7 @@ int intp_serve_build(intp_server s, const intp_grid_desc* desc, const double* va
         r.spline = id;
         r.t_submit = s->now();
         if (s->first_submit < 0) s->first_submit = r.t_submit;
-        s->reqs.emplace(id, std::move(r));
+        s->reqs.emplace(id, std::make_unique<Request>(std::move(r)));
         s->queue.push_back(id);
         ++s->unfinished;
         *request = id;
@@ -473,19 +491,19 @@ int intp_serve_eval(intp_server s, int64_t spline, int64_t npts, const double* c
         auto b = s->reqs.find(spline);
         if (b == s->reqs.end() && spline > 0 && spline < s->next_id)
             return fail(INTP_E_EVICTED, "spline " + std::to_string(spline) + " is not resident");
-        if (b == s->reqs.end() || b->second.kind != INTP_REQ_BUILD)
+        if (b == s->reqs.end() || b->second->kind != INTP_REQ_BUILD)
             return fail(INTP_E_INVALID, "no build request " + std::to_string(spline));
-        const intp_grid_desc& d = b->second.desc;
+        const intp_grid_desc& d = b->second->desc;
         r.orders.assign(orders, orders + static_cast<int64_t>(nout) * d.dim);
         int64_t n = 1;
         for (int i = 0; i < d.dim; ++i) n *= d.n[i];
         const double s_out = (flags & INTP_EVAL_OUT_F32) ? 4 : 8;
-        const double s_tab = b->second.precision == INTP_PREC_F32_REFINED ? 4 : 8;
+        const double s_tab = b->second->precision == INTP_PREC_F32_REFINED ? 4 : 8;
         r.bytes = static_cast<double>(npts) * (d.dim * 8 + nout * s_out) + static_cast<double>(n) * s_tab;
         const int64_t id = s->next_id++;
         r.t_submit = s->now();
         if (s->first_submit < 0) s->first_submit = r.t_submit;
-        s->reqs.emplace(id, std::move(r));
+        s->reqs.emplace(id, std::make_unique<Request>(std::move(r)));
         s->queue.push_back(id);
         ++s->unfinished;
         *request = id;
@@ -499,8 +517,8 @@ int intp_serve_poll(intp_server s, int64_t request, int32_t* state, int32_t* sta
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(request);
     if (it == s->reqs.end()) return fail(INTP_E_INVALID, "no such request");
-    *state = it->second.state;
-    if (status) *status = it->second.status;
+    *state = it->second->state;
+    if (status) *status = it->second->status;
     return INTP_OK;
 }
 
@@ -514,8 +532,8 @@ int intp_serve_wait(intp_server s, int64_t request) {
     s->done.wait(lk, [&] {
         auto it = s->reqs.find(request);
         if (it == s->reqs.end()) return true;
-        st = it->second.status;
-        return it->second.state >= INTP_REQ_DONE;
+        st = it->second->status;
+        return it->second->state >= INTP_REQ_DONE;
     });
     if (st != INTP_OK) return fail(st, "request " + std::to_string(request) + " failed: " + intp_status_string(st));
     return INTP_OK;
@@ -595,7 +613,7 @@ int intp_serve_stats(intp_server s, int32_t reset, intp_serve_counters* out) {
         // forget finished requests, keeping the build records of resident
         // splines (evaluations look their grids up)
         for (auto it = s->reqs.begin(); it != s->reqs.end();) {
-            const Request& r = it->second;
+            const Request& r = *it->second;
             const bool finished = r.state >= INTP_REQ_DONE;
             const bool keep = r.kind == INTP_REQ_BUILD && s->splines.count(it->first);
             if (finished && !keep)

@@ -186,3 +186,43 @@ def workload_batch_mixed(seed: int = SEED, count: int = 4096, queries_per_spline
     values = np.concatenate(vals)
     values = values + normal(seed, STREAM_NOISE, values.size, 1e-2)
     return dict(specs=specs, values=values, points=np.concatenate(pts), point_offsets=np.asarray(offs, np.int64))
+
+
+STREAM_SERVE = 3
+
+
+def workload_serving(seed: int = SEED, rate: float = 1e3, seconds: float = 2.0, count: int = None,
+                     phase: int = 0, recent: int = 16):
+    """Requests of the serving workload: Poisson arrivals at ``rate`` per
+    second for ``seconds`` (or exactly ``count`` requests, all arriving at
+    t = 0 when rate is None).  30% builds: 2D with sides U{32..1024} or
+    3D with sides U{32..192} (even odds), order U{3,4,5}, all dimensions
+    clamped on [0, 1], FP64 or refined FP32 (even odds, 1 sweep).  70%
+    evaluations of log-uniform 1e3..1e6 points, half of them with every
+    first partial next to the value, against one of the ``recent`` latest
+    builds.  Each request: dict(t, kind, ...); uniforms come from stream
+    STREAM_SERVE at offset 8 * (request index + 2^24 * phase)."""
+    n = count if count is not None else int(rate * seconds * 1.5) + 16
+    u = uniform(seed, STREAM_SERVE, 8 * n, offset=8 * (phase << 24)).reshape(n, 8)
+    reqs, builds, t = [], [], 0.0
+    for i in range(n):
+        r = u[i]
+        if rate is not None:
+            t += -math.log1p(-r[0]) / rate
+            if count is None and t > seconds:
+                break
+        if r[1] < 0.3 or not builds:
+            D = 2 if r[2] < 0.5 else 3
+            lo, hi = (32, 1024) if D == 2 else (32, 192)
+            shape = tuple(lo + min(int(r[3 + d] * (hi - lo + 1)), hi - lo) for d in range(D))
+            order = 3 + min(int(r[6] * 3), 2)
+            precision = "f64" if r[7] < 0.5 else "f32_refined"
+            builds.append(len(reqs))
+            reqs.append(dict(t=t if rate is not None else 0.0, kind="build", shape=shape, order=order,
+                             precision=precision))
+        else:
+            npts = int(round(10 ** (3 + 3 * r[2])))
+            target = builds[-1 - min(int(r[3] * recent), len(builds) - 1)]
+            reqs.append(dict(t=t if rate is not None else 0.0, kind="eval", target=target, npts=npts,
+                             derivs=bool(r[4] < 0.5)))
+    return reqs
