@@ -150,6 +150,42 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
 SERVE_RATES = (1e2, 3e2, 1e3, 3e3, 1e4)
 
 
+def measure_small(amd, torch, steps: int):
+    """The small 2D workload: 64x64 order-3 clamped FP64, 1e6 queries of
+    value and d/dx.  Launch-latency bound, so reported in microseconds:
+    build (full constructor), eval call, and the two together, each timed
+    with events over ``steps`` back-to-back repetitions on one stream."""
+    from bsplineinterpolation_amd import datagen
+    w = datagen.workload_2d_64_cubic()
+    dev = torch.device("cuda:0")
+    f = torch.from_numpy(w["f"]).to(dev)
+    q = torch.from_numpy(np.ascontiguousarray(w["queries"].T)).to(dev)
+    out = torch.empty((2, q.shape[0]), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / steps
+
+    g = amd.InterpolationFunction(3, f, periodic=w["periodic"], ranges=w["ranges"])
+    build_us = timed(lambda: amd.InterpolationFunction(3, f, periodic=w["periodic"], ranges=w["ranges"]))
+    eval_us = timed(lambda: g.eval(q, orders=w["orders"], out=out))
+    step_us = timed(lambda: amd.InterpolationFunction(3, f, periodic=w["periodic"], ranges=w["ranges"]).eval(
+        q, orders=w["orders"], out=out))
+    B_solve, B_eval = 2 * 2 * 4096 * 8, 1_000_000 * (16 + 16) + 4096 * 8
+    return dict(workload="2D 64x64 order-3 clamped FP64 build + 1e6 queries x (value, d/dx)",
+                build_us=build_us, eval_us=eval_us, step_us=step_us, points_per_s=1e6 / (step_us * 1e-6),
+                eval_gbs=B_eval / (eval_us * 1e-6) / 1e9, algorithmic_bytes=B_solve + B_eval)
+
+
 def measure_batched(amd, torch, steps: int):
     """The batched form: 4096 mixed 1D/2D/3D FP64 splines (orders 1..5,
     ~25% periodic dimensions), built and evaluated at 256 points each per
@@ -419,9 +455,7 @@ def main():
         print(json.dumps(measure_serving(amd, torch, args.serve_seconds, SERVE_RATES, args.serve_burst)))
         return
     if args.only_forms:
-        forms = measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)
-        forms["batched"] = measure_batched(amd, torch, args.steps)
-        print(json.dumps(forms))
+        print(json.dumps(measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)))
         return
     print("bench: generating inputs (%d^3 mesh, %d queries)" % (n, nq), file=sys.stderr, flush=True)
     w = datagen.workload_3d_256_cubic(n=n, nq=nq)
@@ -528,6 +562,7 @@ def main():
         del keep
         keep = None
         result["forms"] = measure_refined_form(amd, torch, max(3, args.steps // 2), 2, n, nq)
+        result["forms"]["small_2d64"] = measure_small(amd, torch, 20)
         print("bench: batched form (4096 mixed splines)", file=sys.stderr, flush=True)
         result["forms"]["batched"] = measure_batched(amd, torch, max(5, args.steps))
         print("bench: serving loop", file=sys.stderr, flush=True)
