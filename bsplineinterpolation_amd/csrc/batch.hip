@@ -41,6 +41,11 @@ namespace intp_amd {
 
 namespace {
 
+// Rows per substitution batch in the merged kernels (bounds registers:
+// two workgroups per CU).
+constexpr int kMergedRows = 4;
+constexpr int kLineThreads = 64;
+
 template <int BW, bool CYC>
 __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const BatchSolveItem& w, const double* values, double* ctrl,
                                                  int32_t first_pass, unsigned char* tile_bytes, int64_t* dst_base,
@@ -70,7 +75,7 @@ __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const
             }
             sp.shift_last = b.shift[a];
         }
-        solve_tile<BW, CYC, double, double>(sp, w.first_line, tile_bytes, dst_base, src_base);
+        solve_tile<BW, CYC, double, double, kMergedRows>(sp, w.first_line, tile_bytes, dst_base, src_base);
         return;
     }
     // kSolveLanes: one line per lane, in place
@@ -88,7 +93,59 @@ __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const
     sp.bottom = b.bottom[a];
     sp.right = b.right[a];
     const int64_t o = line / inner, ii = line - o * inner;
-    substitute<BW, CYC, double, double>(ctrl + b.offset + o * n * inner + ii, inner, n, sp);
+    substitute<BW, CYC, double, double, kMergedRows>(ctrl + b.offset + o * n * inner + ii, inner, n, sp);
+}
+
+// One-dimensional splines: one spline per 64-thread workgroup, longest
+// first (list1d).  The wave copies the values into the control-point array
+// with the periodic shift (coalesced), then its first lane substitutes the
+// line in place.  Runs on its own stream beside the passes of the
+// multi-dimensional splines: a 1D line is one long sequential chain, and
+// needs a wave, not a workgroup's LDS.
+template <int BW, bool CYC>
+__device__ __forceinline__ void batch_line(const BatchSpline& b, const double* values, double* ctrl) {
+    const int64_t n = b.ep.dim[0].n, shift = b.shift[0];
+    double* x = ctrl + b.offset;
+    const double* f = values + b.offset;
+    for (int64_t j = threadIdx.x; j < n; j += kLineThreads) {
+        int64_t js = j - shift;
+        if (js < 0) js += n;
+        x[j] = f[js];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    SolveParams sp{};
+    sp.lower = b.lower[0];
+    sp.upper = b.upper[0];
+    sp.diag = b.diag[0];
+    sp.bottom = b.bottom[0];
+    sp.right = b.right[0];
+    substitute<BW, CYC, double, double, kMergedRows>(x, 1, n, sp);
+}
+
+__global__ __launch_bounds__(kLineThreads, 4) void batch_line_kernel(const BatchSpline* specs, const int32_t* list1d,
+                                                                  const double* values, double* ctrl) {
+    const BatchSpline& b = specs[list1d[blockIdx.x]];
+    const int kind = 2 * b.band[0] + (b.ep.dim[0].periodic ? 1 : 0);
+#define INTP_LINE_CASE(BW, CYC) \
+    case 2 * BW + CYC:          \
+        batch_line<BW, CYC>(b, values, ctrl); \
+        break;
+    switch (kind) {
+        INTP_LINE_CASE(0, false)
+        INTP_LINE_CASE(0, true)
+        INTP_LINE_CASE(1, false)
+        INTP_LINE_CASE(1, true)
+        INTP_LINE_CASE(2, false)
+        INTP_LINE_CASE(2, true)
+        INTP_LINE_CASE(3, false)
+        INTP_LINE_CASE(3, true)
+        INTP_LINE_CASE(4, false)
+        INTP_LINE_CASE(5, false)
+        INTP_LINE_CASE(6, false)
+        default: break;  // band widths are checked on the host
+    }
+#undef INTP_LINE_CASE
 }
 
 // One launch per pass: workgroup g solves items g, g + gridDim.x, ...
@@ -129,6 +186,7 @@ __global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSp
 }
 
 constexpr int kMaxBatchBand = 6;
+constexpr int kEvalStreams = 4;
 
 // LDS budget of one solve workgroup (two per CU).
 constexpr int64_t kBatchTileBudget = 80 * 1024 - 2 * kMaxTileLines * 8;
@@ -181,6 +239,13 @@ struct intp_batch_s {
     std::vector<BatchSolveItem> solve_host;
     BatchSolveItem* solve_items = nullptr;
     int64_t solve_cap = 0;
+    std::vector<int32_t> list1d_host;  // the 1D splines, longest first
+    int32_t* list1d = nullptr;
+    hipStream_t side = nullptr;        // runs the 1D splines beside the passes
+    hipEvent_t fork = nullptr, join = nullptr;
+    // the (dimension, order) eval classes run side by side on these
+    hipStream_t eval_side[kEvalStreams] = {};
+    hipEvent_t eval_fork = nullptr, eval_join[kEvalStreams] = {};
     std::vector<SolveGroup> solve_groups;
     // eval work for the last point layout
     std::vector<BatchEvalItem> eval_host;
@@ -207,6 +272,21 @@ struct intp_batch_s {
         release_after(device, specs, uses);
         release_after(device, ctrl, uses);
         release_after(device, solve_items, uses);
+        release_after(device, list1d, uses);
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        for (int i = 0; i < kEvalStreams; ++i) {
+            if (eval_side[i]) {
+                (void)hipStreamSynchronize(eval_side[i]);
+                (void)hipStreamDestroy(eval_side[i]);
+            }
+            if (eval_join[i]) (void)hipEventDestroy(eval_join[i]);
+        }
+        if (eval_fork) (void)hipEventDestroy(eval_fork);
         release_after(device, eval_items, uses);
     }
 };
@@ -225,7 +305,7 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
             const int64_t lpb = tile_fit(b->setups[static_cast<size_t>(s) * kMaxDim + d].n, b->tune.tile_cap);
             int64_t inner = 1;
             for (int e = d + 1; e < D; ++e) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + e].n;
-            if (lpb == 0 && inner < 64)
+            if (lpb == 0 && inner < 64 && D > 1)
                 return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": line too long for a batch");
             bs.lines_per_block[d] = static_cast<int32_t>(lpb);
         }
@@ -253,7 +333,9 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
             int64_t inner = 1;
             for (int d = a + 1; d < D; ++d) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + d].n;
             const int64_t cost = ds.n * row_cost(ds);
-            if (inner >= 64) {
+            if (D == 1) {
+                continue;  // batch_line_kernel, beside the passes
+            } else if (inner >= 64) {
                 for (int64_t first = 0; first < nlines; first += kTileThreads)
                     pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveLanes), first}});
             } else {
@@ -329,6 +411,16 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
 }
 
 int launch_build(intp_batch b, const double* values, hipStream_t stream) {
+    const bool lines = !b->list1d_host.empty();
+    if (lines) {
+        // fork: the 1D splines on the side stream, joined before returning
+        HIP_TRY(hipEventRecord(b->fork, stream), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0), "hipStreamWaitEvent");
+        hipLaunchKernelGGL(batch_line_kernel, dim3(static_cast<unsigned>(b->list1d_host.size())), dim3(kLineThreads),
+                           0, b->side, b->specs, b->list1d, values, b->ctrl);
+        HIP_TRY(hipGetLastError(), "batch line launch");
+        HIP_TRY(hipEventRecord(b->join, b->side), "hipEventRecord");
+    }
     for (const SolveGroup& g : b->solve_groups) {
         const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
         hipLaunchKernelGGL(batch_solve_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kTileThreads), g.lds, stream,
@@ -336,19 +428,38 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream) {
                            g.pass == 0 ? 1 : 0);
         HIP_TRY(hipGetLastError(), "batch solve launch");
     }
+    if (lines) HIP_TRY(hipStreamWaitEvent(stream, b->join, 0), "hipStreamWaitEvent");
     return INTP_OK;
 }
 
 int launch_eval_work(intp_batch b, const double* coords, double* out, hipStream_t stream) {
+    // each class is a small launch: fork them over the side streams so that
+    // they run together, and join before returning
+    if (!b->eval_fork) {
+        HIP_TRY(hipEventCreateWithFlags(&b->eval_fork, hipEventDisableTiming), "hipEventCreate");
+        for (int i = 0; i < kEvalStreams; ++i) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->eval_side[i], hipStreamNonBlocking), "hipStreamCreate");
+            HIP_TRY(hipEventCreateWithFlags(&b->eval_join[i], hipEventDisableTiming), "hipEventCreate");
+        }
+    }
+    const int nside = static_cast<int>(std::min<size_t>(kEvalStreams, b->eval_groups.size()));
+    HIP_TRY(hipEventRecord(b->eval_fork, stream), "hipEventRecord");
+    for (int i = 0; i < nside; ++i) HIP_TRY(hipStreamWaitEvent(b->eval_side[i], b->eval_fork, 0), "hipStreamWaitEvent");
+    int next = 0;
     for (const EvalGroup& g : b->eval_groups) {
+        hipStream_t s = b->eval_side[next++ % nside];
         BatchEvalArgs a{b->specs, b->eval_items + g.offset, static_cast<int32_t>(g.count), coords, out};
         const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.eval_blocks_per_cu));
         hipError_t e = hipErrorNotSupported;
-        if (g.dim == 1) e = batch_eval_dispatch_d1(g.order, a, blocks, b->tune.eval_threads, stream);
-        if (g.dim == 2) e = batch_eval_dispatch_d2(g.order, a, blocks, b->tune.eval_threads, stream);
-        if (g.dim == 3) e = batch_eval_dispatch_d3(g.order, a, blocks, b->tune.eval_threads, stream);
+        if (g.dim == 1) e = batch_eval_dispatch_d1(g.order, a, blocks, b->tune.eval_threads, s);
+        if (g.dim == 2) e = batch_eval_dispatch_d2(g.order, a, blocks, b->tune.eval_threads, s);
+        if (g.dim == 3) e = batch_eval_dispatch_d3(g.order, a, blocks, b->tune.eval_threads, s);
         if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "no batched kernel for this dimension/order");
         HIP_TRY(e, "batch eval launch");
+    }
+    for (int i = 0; i < nside; ++i) {
+        HIP_TRY(hipEventRecord(b->eval_join[i], b->eval_side[i]), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(stream, b->eval_join[i], 0), "hipStreamWaitEvent");
     }
     return INTP_OK;
 }
@@ -555,6 +666,7 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
             bs.bottom[d] = dptr(w.bottom, ds.lu.bottom.size());
             bs.right[d] = dptr(w.right, ds.lu.right.size());
             bs.shift[d] = ds.periodic ? (K / 2) % ds.n : 0;
+            bs.band[d] = ds.lu.p;
         }
         ep.ctrl = b->ctrl + bs.offset;
         ep.cs_dim = 1;
@@ -563,6 +675,27 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
         ep.os_pt = 1;
         for (int i = 0; i <= kMaxDim; ++i) ep.slot[i] = -1;
         ep.out_index = 0;
+    }
+    // the 1D splines, longest first (they run beside the passes)
+    for (int s = 0; s < count; ++s)
+        if (b->dims[static_cast<size_t>(s)] == 1) {
+            const LineFactors& lu = b->setups[static_cast<size_t>(s) * kMaxDim].lu;
+            if (lu.p > (lu.cyclic ? 3 : kMaxBatchBand))
+                return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
+            b->list1d_host.push_back(s);
+        }
+    std::stable_sort(b->list1d_host.begin(), b->list1d_host.end(), [&](int32_t x, int32_t y) {
+        return b->setups[static_cast<size_t>(x) * kMaxDim].n > b->setups[static_cast<size_t>(y) * kMaxDim].n;
+    });
+    if (!b->list1d_host.empty()) {
+        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->list1d), b->list1d_host.size() * sizeof(int32_t), stream),
+                "hipMallocAsync (batch work)");
+        HIP_TRY(hipMemcpyAsync(b->list1d, b->list1d_host.data(), b->list1d_host.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+        HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
+        HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+        HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
     }
     const int rc = make_solve_work(b.get(), stream);
     if (rc != INTP_OK) return rc;

@@ -24,6 +24,7 @@ struct BatchSpline {
     int64_t shift[kMaxDim];
     int32_t dim, order;
     int32_t lines_per_block[kMaxDim];  // LDS tile of the solve along each axis
+    int32_t band[kMaxDim];             // band width of each axis's LU
 };
 
 // Work of one solve workgroup.  kind & 15 = 2 * band width + cyclic

@@ -183,6 +183,7 @@ __device__ __forceinline__ void run_backward(BackwardBatch<BW, CYC, C, B>& r, P*
 
 // Rows per batch: two batches are in registers at once (the one being
 // substituted and the next one loading), so wide cyclic bands use 4.
+// Kernels that merge many band widths pass 4 to bound their registers.
 template <int BW, bool CYC>
 constexpr int batch_rows() {
     return BW * (CYC ? 2 : 1) <= 4 ? kBatch : kBatch / 2;
@@ -194,9 +195,8 @@ constexpr int batch_rows() {
 // substituted, so one lane's chain of dependent rows does not wait for
 // memory every batch.  Each row receives the same updates in the same
 // order as without pipelining.
-template <int BW, bool CYC, typename C, typename P>
+template <int BW, bool CYC, typename C, typename P, int B = batch_rows<BW, CYC>()>
 __device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
-    constexpr int B = batch_rows<BW, CYC>();
     LineState<BW, CYC, C> st;
 #pragma unroll
     for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = C(0);
@@ -254,7 +254,7 @@ __host__ __device__ __forceinline__ int64_t tile_stride(int64_t n) { return n | 
 // One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS in
 // the compute type C; the table holds T.  With sp.accum set, the solved
 // lines are added into accum instead of being stored to x.
-template <int BW, bool CYC, typename T, typename C>
+template <int BW, bool CYC, typename T, typename C, int B = batch_rows<BW, CYC>()>
 __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first, unsigned char* tile_bytes,
                                            int64_t* dst_base, int64_t* src_base) {
     C* tile = reinterpret_cast<C*>(tile_bytes);
@@ -346,7 +346,7 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
     }
     __syncthreads();
 
-    if (tid < nl) substitute<BW, CYC, C, C>(tile + tid * stride, 1, n, sp);
+    if (tid < nl) substitute<BW, CYC, C, C, B>(tile + tid * stride, 1, n, sp);
     __syncthreads();
 
     // ---- store ----
