@@ -150,6 +150,42 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
 SERVE_RATES = (1e2, 3e2, 1e3, 3e3, 1e4)
 
 
+def measure_refined_headline(amd, torch, steps: int, warmup: int, w, f_d, q_d, out64, f64_ms: float, sweeps: int = 1):
+    """The refined-FP32 form on the headline workload (3D 256^3 cubic,
+    value + 3 partials at 2^26 points, FP32 results): the same step as the
+    FP64 headline, timed the same way, with max/RMS relative error of every
+    output against the FP64 results of the headline run."""
+    dev = q_d.device
+    out32 = torch.empty((out64.shape[0], out64.shape[1]), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        s = amd.InterpolationFunction(3, f_d, periodic=w["periodic"], ranges=w["ranges"], precision="f32_refined",
+                                      sweeps=sweeps)
+        s.eval(q_d, w["orders"], out=out32.t())
+        return s
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    errs = {}
+    names = ["value", "d/dx", "d/dy", "d/dz"]
+    for o in range(out64.shape[1]):
+        ref = out64[:, o]
+        d = out32[:, o].double() - ref
+        errs[names[o]] = {"max": float(d.abs().max() / ref.abs().max()),
+                          "rms": float(torch.sqrt((d * d).mean() / (ref * ref).mean()))}
+    del stream
+    return {"workload": "the headline step (3D 256^3 cubic build + 2^26 x value, d/dx, d/dy, d/dz) in the refined-FP32 "
+                        "form, %d sweep(s), FP32 results" % sweeps,
+            "ms_per_step": ms, "f64_ms_per_step": f64_ms, "speedup_vs_f64": f64_ms / ms, "error_vs_f64": errs}
+
+
 def measure_small(amd, torch, steps: int):
     """The small 2D workload: 64x64 order-3 clamped FP64, 1e6 queries of
     value and d/dx.  Launch-latency bound, so reported in microseconds:
@@ -569,6 +605,9 @@ def main():
         keep = None
         result["forms"] = measure_refined_form(amd, torch, max(3, args.steps // 2), 2, n, nq)
         result["forms"]["small_2d64"] = measure_small(amd, torch, 20)
+        print("bench: refined-FP32 form on the headline workload", file=sys.stderr, flush=True)
+        result["forms"]["refined_headline"] = measure_refined_headline(amd, torch, max(3, args.steps // 2), 2, w, f_d,
+                                                                       q_d, out_d, ms_per_step)
         print("bench: batched form (4096 mixed splines)", file=sys.stderr, flush=True)
         result["forms"]["batched"] = measure_batched(amd, torch, max(5, args.steps))
         print("bench: serving loop", file=sys.stderr, flush=True)
