@@ -103,3 +103,30 @@ def test_gpu_batch_rejects_bad_specs(amd):
         amd.Batch([dict(order=3, shape=(10,), ranges=[(0, 1)]), dict(order=9, shape=(10,), ranges=[(0, 1)])])
     with pytest.raises(amd.IntpError):
         amd.Batch([dict(order=4, shape=(4,), periodic=[True], ranges=[(0, 1)])])
+
+
+def test_gpu_batch_long_lines_and_singletons(amd):
+    """1D splines longer than an LDS tile (solved in place by the line
+    kernel), periodic and clamped, beside small 2D/3D ones; equal to the
+    single-spline path."""
+    import torch
+    rng = np.random.default_rng(21)
+    specs = [dict(order=5, shape=(20000,), periodic=[True], ranges=[(0.0, 1.0)]),
+             dict(order=4, shape=(15000,), periodic=[False], ranges=[(0.0, 2.0)]),
+             dict(order=1, shape=(2,), periodic=[False], ranges=[(0.0, 1.0)]),
+             dict(order=3, shape=(5, 70), periodic=[False, True], ranges=[(0.0, 1.0), (0.0, 1.0)]),
+             dict(order=2, shape=(9, 8, 66), periodic=[True, False, False], ranges=[(0.0, 1.0)] * 3)]
+    b = amd.Batch(specs)
+    vals = rng.standard_normal(b.total)
+    q = 300
+    pts = np.zeros((q * len(specs), 3))
+    pts[:, :] = rng.random((q * len(specs), 3)) * 0.999
+    pts[q:2 * q, 0] *= 2.0
+    po = np.arange(len(specs) + 1) * q
+    out = b.run(torch.from_numpy(vals).cuda(), torch.from_numpy(pts).cuda(), po).cpu().numpy()
+    for s, sp in enumerate(specs):
+        f = vals[b.offsets[s]:b.offsets[s + 1]].reshape(sp["shape"])
+        g = amd.InterpolationFunction(sp["order"], f, periodic=sp["periodic"], ranges=sp["ranges"])
+        assert np.array_equal(b.control_points(s), g.control_points().ravel()), s
+        D = len(sp["shape"])
+        assert np.array_equal(out[po[s]:po[s + 1]], g.eval(pts[po[s]:po[s + 1], :D])[0]), s

@@ -144,3 +144,52 @@ def test_gpu_serve_counters(amd):
     assert again["eval"]["completed"] == 0 and again["resident_splines"] == 1
     srv.close()
     torch.cuda.synchronize()
+
+
+def test_gpu_serve_concurrent_clients_and_nonuniform_grids(amd):
+    """Several client threads submit builds (uniform and non-uniform grids,
+    both precisions) and evaluations at once; each result equals the
+    direct calls."""
+    import threading
+    import torch
+    srv = amd.Server(streams=4)
+    rng = np.random.default_rng(9)
+    jobs = []
+    for t in range(4):
+        n0, n1 = 30 + 7 * t, 25 + 5 * t
+        xs = np.sort(rng.random(n0)) * 2.0
+        xs[0], xs[-1] = 0.0, 2.0
+        f = torch.from_numpy(rng.standard_normal((n0, n1))).cuda()
+        pts = np.column_stack([rng.random(2000) * 2.0, rng.random(2000), np.zeros(2000)])
+        jobs.append(dict(order=3 + t % 3, f=f, coords=[xs, None], ranges=[None, (0.0, 1.0)],
+                         precision="f64" if t % 2 == 0 else "f32_refined", pts=torch.from_numpy(pts).cuda()))
+    results = [None] * len(jobs)
+    errors = []
+
+    def client(i):
+        try:
+            j = jobs[i]
+            sid = srv.build(j["order"], j["f"], ranges=j["ranges"], coords=j["coords"], precision=j["precision"],
+                            sweeps=2)
+            outs = [srv.eval(sid, j["pts"], [(0, 0), (1, 0), (0, 1)]) for _ in range(3)]
+            res = [srv.wait(r).cpu().numpy() for r in outs]
+            srv.wait(sid)
+            results[i] = (srv.spline(sid).control_points(), res)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=client, args=(i,)) for i in range(len(jobs))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for i, j in enumerate(jobs):
+        g = amd.InterpolationFunction(j["order"], j["f"], ranges=j["ranges"], coords=j["coords"],
+                                      precision=j["precision"], sweeps=2)
+        ctrl, res = results[i]
+        assert np.array_equal(ctrl, g.control_points()), i
+        direct = g.eval(j["pts"][:, :2].contiguous(), orders=[(0, 0), (1, 0), (0, 1)]).cpu().numpy()
+        for r in res:
+            assert np.array_equal(r, direct), i
+    srv.close()
