@@ -255,6 +255,19 @@ def measure_batched(amd, torch, steps: int):
         stream.synchronize()
         wall_ms = (time.perf_counter() - t0) * 1e3 / steps
     run_ms = e0.elapsed_time(e1) / steps
+    # the same batch with long 1D lines split into parallel segments
+    # (INTP_BATCH_SPLIT_LINES: within 1e-12 of the sequential solve)
+    seq_out = out.clone()
+    with torch.cuda.stream(stream):
+        b.run(vals, pts, po, out=out, graph=True, split_lines=True)  # capture
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            b.run(vals, pts, po, out=out, graph=True, split_lines=True)
+        e1.record(stream)
+        stream.synchronize()
+    split_ms = e0.elapsed_time(e1) / steps
+    d = (out - seq_out).abs().max().item() / seq_out.abs().max().item()
     N = int(b.total)
     Q = int(po[-1])
     dims = [len(sp["shape"]) for sp in w["specs"]]
@@ -268,6 +281,9 @@ def measure_batched(amd, torch, steps: int):
             "points_per_s": Q / (run_ms * 1e-3), "algorithmic_bytes": b_solve + b_eval,
             "gbs": (b_solve + b_eval) / (run_ms * 1e-3) / 1e9, "create_ms": create_ms, "autotune_ms": tune_ms,
             "tuning": b.tuning().split(" | ")[0], "graph": True,
+            "split_lines": {"run_ms": split_ms, "points_per_s": Q / (split_ms * 1e-3),
+                            "gbs": (b_solve + b_eval) / (split_ms * 1e-3) / 1e9,
+                            "max_rel_diff_vs_sequential": d},
             "parity": "control points and values bit-identical to the single-spline path (tests/test_gpu_batch.py)"}
 
 

@@ -378,14 +378,17 @@ class Batch:
         return out
 
     def run(self, values, coords, point_offsets, out=None, graph: bool = False, autotune: bool = False,
-            stream=None):
+            split_lines: bool = False, stream=None):
         """Build + eval (device tensors); ``graph`` replays one captured
-        hipGraph while buffers and layout stay the same."""
+        hipGraph while buffers and layout stay the same; ``split_lines``
+        solves long 1D lines in parallel segments (INTP_BATCH_SPLIT_LINES:
+        within 1e-12 of the sequential solve, not bit-identical)."""
         import torch
         po, pp = self._offsets(point_offsets)
         if out is None:
             out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
-        flags = (_lib.INTP_BATCH_GRAPH if graph else 0) | (_lib.INTP_BATCH_AUTOTUNE if autotune else 0)
+        flags = ((_lib.INTP_BATCH_GRAPH if graph else 0) | (_lib.INTP_BATCH_AUTOTUNE if autotune else 0) |
+                 (_lib.INTP_BATCH_SPLIT_LINES if split_lines else 0))
         st = _stream_handle(stream) if stream is not None else _current_stream()
         check(_lib.load().intp_batch_run(self.h, C.cast(values.data_ptr(), _dp), C.cast(coords.data_ptr(), _dp), pp,
                                          C.cast(out.data_ptr(), _dp), flags, st))

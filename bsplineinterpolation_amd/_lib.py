@@ -31,6 +31,7 @@ INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
 INTP_BATCH_GRAPH = 1
 INTP_BATCH_AUTOTUNE = 2
+INTP_BATCH_SPLIT_LINES = 4
 INTP_REQ_BUILD = 0
 INTP_REQ_EVAL = 1
 INTP_REQ_QUEUED = 0

@@ -45,6 +45,9 @@ namespace {
 // two workgroups per CU).
 constexpr int kMergedRows = 4;
 constexpr int kLineThreads = 64;
+constexpr int kMaxBatchBand = 6;
+// 1D splines of at least this many points are split (INTP_BATCH_SPLIT_LINES)
+constexpr int64_t kSplitMinRows = 256;
 
 template <int BW, bool CYC>
 __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const BatchSolveItem& w, const double* values, double* ctrl,
@@ -107,6 +110,16 @@ __device__ __forceinline__ void batch_line(const BatchSpline& b, const double* v
     const int64_t n = b.ep.dim[0].n, shift = b.shift[0];
     double* x = ctrl + b.offset;
     const double* f = values + b.offset;
+    if constexpr (BW == 0) {
+        // a diagonal system: each row is one division, every lane takes rows
+        const double* __restrict__ diag = b.diag[0];
+        for (int64_t j = threadIdx.x; j < n; j += kLineThreads) {
+            int64_t js = j - shift;
+            if (js < 0) js += n;
+            x[j] = f[js] / diag[j];
+        }
+        return;
+    }
     for (int64_t j = threadIdx.x; j < n; j += kLineThreads) {
         int64_t js = j - shift;
         if (js < 0) js += n;
@@ -123,9 +136,191 @@ __device__ __forceinline__ void batch_line(const BatchSpline& b, const double* v
     substitute<BW, CYC, double, double, kMergedRows>(x, 1, n, sp);
 }
 
+// Split solve of one long 1D line (INTP_BATCH_SPLIT_LINES).  The first
+// H = n - 2*BW - 8 rows are cut into S <= 63 segments, one per lane.  The
+// forward sweep is linear in the BW values entering a segment, so each
+// lane first substitutes its segment from a zero state (y0), one lane then
+// chains the states through the segments (state out = y0's last BW rows +
+// gf x state in, staged in LDS), and each lane adds gf x (its state in) to
+// its rows.  The last rows, where the cyclic strips and corners act, run
+// sequentially with the exact state, the strip sums of the head taken per
+// segment.  The backward sweep mirrors this from the right (gb), with the
+// corner values as constants.  Same system, different rounding order.
+struct SplitShared {
+    double state[kLineThreads][kMaxBatchBand];  // state entering each segment
+    double y[kLineThreads][kMaxBatchBand];      // a segment's zero-state boundary rows
+    double g[kLineThreads][kMaxBatchBand][kMaxBatchBand];  // and their responses
+    double vec[2][kMaxBatchBand];               // corners; backward state from the tail
+};
+
+template <int BW, bool CYC>
+__device__ __forceinline__ void batch_line_split(const BatchSpline& b, const double* values, double* ctrl,
+                                                 SplitShared& sh) {
+#pragma clang fp contract(off)
+    constexpr int kB = BW > 0 ? BW : 1;
+    const int64_t n = b.ep.dim[0].n, shift = b.shift[0];
+    double* x = ctrl + b.offset;
+    const double* f = values + b.offset;
+    for (int64_t j = threadIdx.x; j < n; j += kLineThreads) {
+        int64_t js = j - shift;
+        if (js < 0) js += n;
+        x[j] = f[js];
+    }
+    __syncthreads();
+    const int S = b.nseg;
+    const int64_t H = n - 2 * BW - 8;
+    const int lane = threadIdx.x;
+    const int64_t a = static_cast<int64_t>(lane) * H / S, e = static_cast<int64_t>(lane + 1) * H / S;
+    SolveParams sp{};
+    sp.lower = b.lower[0];
+    sp.upper = b.upper[0];
+    sp.diag = b.diag[0];
+    sp.bottom = b.bottom[0];
+    sp.right = b.right[0];
+    const double* __restrict__ bottom = b.bottom[0];
+    const double* __restrict__ gf = b.gf;
+    const double* __restrict__ gb = b.gb;
+    const int32_t z = lane_zero();
+    // forward, zero state entering each segment (the head has no strip rows)
+    if (lane < S) {
+        LineState<BW, false, double> st0;
+#pragma unroll
+        for (int m = 0; m <= BW; ++m) st0.hist[m] = st0.nxt[m] = st0.corner[m] = st0.acc[m] = 0.0;
+        forward_range<BW, false, double, double, kMergedRows>(x, 1, n, a, e, sp, st0, z);
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            sh.y[lane][q] = x[e - 1 - q];
+#pragma unroll
+            for (int m = 0; m < BW; ++m) sh.g[lane][q][m] = gf[(e - 1 - q) * BW + m];
+        }
+    }
+    __syncthreads();
+    // chain the states through the segments
+    if (lane == 0) {
+        double h[kB] = {};
+        for (int s = 0; s < S; ++s) {
+            double out[kB];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) {
+                sh.state[s][q] = h[q];
+                double v = sh.y[s][q];
+#pragma unroll
+                for (int m = 0; m < BW; ++m) v = v + sh.g[s][q][m] * h[m];
+                out[q] = v;
+            }
+#pragma unroll
+            for (int m = 0; m < BW; ++m) h[m] = out[m];
+        }
+    }
+    __syncthreads();
+    if (lane < S) {
+        if (lane > 0) {
+            double hin[kB];
+#pragma unroll
+            for (int m = 0; m < BW; ++m) hin[m] = sh.state[lane][m];
+#pragma unroll 4
+            for (int64_t j = a; j < e; ++j) {
+                double v = x[j];
+#pragma unroll
+                for (int m = 0; m < BW; ++m) v = v + gf[j * BW + m] * hin[m];
+                x[j] = v;
+            }
+        }
+        // strip sums of the head rows, per segment (cyclic matrices)
+        if constexpr (CYC) {
+            double part[kB] = {};
+#pragma unroll 4
+            for (int64_t j = a; j < e; ++j) {
+                const double v = x[j];
+#pragma unroll
+                for (int q = 0; q < BW; ++q) part[q] = part[q] - bottom[q * n + j] * v;
+            }
+#pragma unroll
+            for (int q = 0; q < BW; ++q) sh.y[lane][q] = part[q];
+        }
+    }
+    __syncthreads();
+    // the last rows, sequentially with the exact state; then the first rows
+    // of the backward sweep, which fix the corner values
+    if (lane == 0) {
+        LineState<BW, CYC, double> st;
+#pragma unroll
+        for (int m = 0; m <= BW; ++m) st.hist[m] = st.nxt[m] = st.corner[m] = st.acc[m] = 0.0;
+#pragma unroll
+        for (int m = 0; m < BW; ++m) st.hist[m] = x[H - 1 - m];
+        if constexpr (CYC) {
+#pragma unroll
+            for (int q = 0; q < BW; ++q) {
+                double acc = x[n - BW + q];
+                for (int s = 0; s < S; ++s) acc = acc + sh.y[s][q];
+                st.acc[q] = acc;
+            }
+        }
+        forward_range<BW, CYC, double, double, kMergedRows>(x, 1, n, H, n, sp, st, z);
+        backward_range<BW, CYC, double, double, kMergedRows>(x, 1, n, H, n, sp, st, z);
+#pragma unroll
+        for (int m = 0; m < BW; ++m) {
+            sh.vec[0][m] = st.corner[m];
+            sh.vec[1][m] = st.nxt[m];  // x[H + m]
+        }
+    }
+    __syncthreads();
+    // backward, zero state entering each segment from the right
+    if (lane < S) {
+        LineState<BW, CYC, double> st0;
+#pragma unroll
+        for (int m = 0; m <= BW; ++m) st0.hist[m] = st0.nxt[m] = st0.corner[m] = st0.acc[m] = 0.0;
+#pragma unroll
+        for (int c = 0; c < BW; ++c) st0.corner[c] = sh.vec[0][c];
+        backward_range<BW, CYC, double, double, kMergedRows>(x, 1, n, a, e, sp, st0, z);
+#pragma unroll
+        for (int q = 0; q < BW; ++q) {
+            sh.y[lane][q] = x[a + q];
+#pragma unroll
+            for (int m = 0; m < BW; ++m) sh.g[lane][q][m] = gb[(a + q) * BW + m];
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double h[kB];
+#pragma unroll
+        for (int m = 0; m < BW; ++m) h[m] = sh.vec[1][m];
+        for (int s = S - 1; s >= 0; --s) {
+            double out[kB];
+#pragma unroll
+            for (int q = 0; q < BW; ++q) {
+                sh.state[s][q] = h[q];
+                double v = sh.y[s][q];
+#pragma unroll
+                for (int m = 0; m < BW; ++m) v = v + sh.g[s][q][m] * h[m];
+                out[q] = v;
+            }
+#pragma unroll
+            for (int m = 0; m < BW; ++m) h[m] = out[m];
+        }
+    }
+    __syncthreads();
+    if (lane < S) {
+        double nin[kB];
+#pragma unroll
+        for (int m = 0; m < BW; ++m) nin[m] = sh.state[lane][m];
+#pragma unroll 4
+        for (int64_t j = a; j < e; ++j) {
+            double v = x[j];
+#pragma unroll
+            for (int m = 0; m < BW; ++m) v = v + gb[j * BW + m] * nin[m];
+            x[j] = v;
+        }
+    }
+}
+
+// skip_split: the lines with segment responses are solved by
+// batch_line_split_kernel instead.
 __global__ __launch_bounds__(kLineThreads, 4) void batch_line_kernel(const BatchSpline* specs, const int32_t* list1d,
-                                                                  const double* values, double* ctrl) {
+                                                                  const double* values, double* ctrl,
+                                                                  int32_t skip_split) {
     const BatchSpline& b = specs[list1d[blockIdx.x]];
+    if (skip_split && b.nseg > 0) return;
     const int kind = 2 * b.band[0] + (b.ep.dim[0].periodic ? 1 : 0);
 #define INTP_LINE_CASE(BW, CYC) \
     case 2 * BW + CYC:          \
@@ -134,6 +329,31 @@ __global__ __launch_bounds__(kLineThreads, 4) void batch_line_kernel(const Batch
     switch (kind) {
         INTP_LINE_CASE(0, false)
         INTP_LINE_CASE(0, true)
+        INTP_LINE_CASE(1, false)
+        INTP_LINE_CASE(1, true)
+        INTP_LINE_CASE(2, false)
+        INTP_LINE_CASE(2, true)
+        INTP_LINE_CASE(3, false)
+        INTP_LINE_CASE(3, true)
+        INTP_LINE_CASE(4, false)
+        INTP_LINE_CASE(5, false)
+        INTP_LINE_CASE(6, false)
+        default: break;  // band widths are checked on the host
+    }
+#undef INTP_LINE_CASE
+}
+
+// One split line per workgroup (list: the splines with segment responses).
+__global__ __launch_bounds__(kLineThreads) void batch_line_split_kernel(const BatchSpline* specs, const int32_t* list,
+                                                                        const double* values, double* ctrl) {
+    __shared__ SplitShared sh;
+    const BatchSpline& b = specs[list[blockIdx.x]];
+    const int kind = 2 * b.band[0] + (b.ep.dim[0].periodic ? 1 : 0);
+#define INTP_LINE_CASE(BW, CYC) \
+    case 2 * BW + CYC:          \
+        batch_line_split<BW, CYC>(b, values, ctrl, sh); \
+        break;
+    switch (kind) {
         INTP_LINE_CASE(1, false)
         INTP_LINE_CASE(1, true)
         INTP_LINE_CASE(2, false)
@@ -185,7 +405,6 @@ __global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSp
     }
 }
 
-constexpr int kMaxBatchBand = 6;
 constexpr int kEvalStreams = 4;
 
 // LDS budget of one solve workgroup (two per CU).
@@ -241,8 +460,13 @@ struct intp_batch_s {
     int64_t solve_cap = 0;
     std::vector<int32_t> list1d_host;  // the 1D splines, longest first
     int32_t* list1d = nullptr;
+    double* split_tables = nullptr;    // segment responses of the split 1D lines
+    std::vector<int32_t> split_host;   // the split lines, longest first
+    int32_t* split_list = nullptr;
+    bool split_ready = false;
     hipStream_t side = nullptr;        // runs the 1D splines beside the passes
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t side2 = nullptr;       // the split 1D lines, beside both
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
     // the (dimension, order) eval classes run side by side on these
     hipStream_t eval_side[kEvalStreams] = {};
     hipEvent_t eval_fork = nullptr, eval_join[kEvalStreams] = {};
@@ -258,6 +482,7 @@ struct intp_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* graph_key[4] = {};
+    bool graph_split = false;
     std::vector<int64_t> graph_layout;
     Tuning tune;
     bool tuned = false;
@@ -273,12 +498,19 @@ struct intp_batch_s {
         release_after(device, ctrl, uses);
         release_after(device, solve_items, uses);
         release_after(device, list1d, uses);
+        release_after(device, split_tables, uses);
+        release_after(device, split_list, uses);
         if (side) {
             (void)hipStreamSynchronize(side);
             (void)hipStreamDestroy(side);
         }
+        if (side2) {
+            (void)hipStreamSynchronize(side2);
+            (void)hipStreamDestroy(side2);
+        }
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (join2) (void)hipEventDestroy(join2);
         for (int i = 0; i < kEvalStreams; ++i) {
             if (eval_side[i]) {
                 (void)hipStreamSynchronize(eval_side[i]);
@@ -410,14 +642,94 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
     return INTP_OK;
 }
 
-int launch_build(intp_batch b, const double* values, hipStream_t stream) {
+// Segment responses of every long 1D line (INTP_BATCH_SPLIT_LINES), from
+// the same factors and recurrences as the device: gf[j][m] is row j's
+// forward value when only the m-th value entering its segment is 1 (zero
+// right-hand side), gb[j][m] the backward one.  Computed and uploaded on
+// first use.
+int prepare_split(intp_batch b, hipStream_t stream) {
+    if (b->split_ready) return INTP_OK;
+    std::vector<double> tab;
+    std::vector<std::pair<int32_t, size_t>> at;  // spline, offset of its gf (gb follows)
+    for (int32_t s : b->list1d_host) {
+        const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim];
+        const LineFactors& lu = ds.lu;
+        const int BW = lu.p;
+        const int64_t n = ds.n, H = n - 2 * BW - 8;
+        if (n < kSplitMinRows || BW < 1 || H < 64) continue;
+        const int S = static_cast<int>(std::min<int64_t>(63, std::max<int64_t>(2, H / 32)));
+        const size_t off = tab.size();
+        tab.resize(off + 2 * static_cast<size_t>(H) * BW, 0.0);
+        double* gf = tab.data() + off;
+        double* gb = gf + static_cast<size_t>(H) * BW;
+        for (int sg = 0; sg < S; ++sg) {
+            const int64_t a = static_cast<int64_t>(sg) * H / S, e = static_cast<int64_t>(sg + 1) * H / S;
+            for (int m = 0; m < BW; ++m) {
+                double hist[8] = {}, nx[8] = {};
+                hist[m] = 1.0;
+                for (int64_t j = a; j < e; ++j) {
+                    double v = 0.0;
+                    for (int k = BW; k >= 1; --k) v = v - lu.lower[static_cast<size_t>(j * BW + k - 1)] * hist[k - 1];
+                    gf[j * BW + m] = v;
+                    for (int k = BW; k >= 2; --k) hist[k - 1] = hist[k - 2];
+                    hist[0] = v;
+                }
+                nx[m] = 1.0;
+                for (int64_t j = e - 1; j >= a; --j) {
+                    double v = 0.0;
+                    for (int k = BW; k >= 1; --k) v = v - lu.upper[static_cast<size_t>(j * BW + k - 1)] * nx[k - 1];
+                    v = v / lu.diag[static_cast<size_t>(j)];
+                    gb[j * BW + m] = v;
+                    for (int k = BW; k >= 2; --k) nx[k - 1] = nx[k - 2];
+                    nx[0] = v;
+                }
+            }
+        }
+        at.push_back({s, off});
+        b->split_host.push_back(s);
+        b->host_specs[static_cast<size_t>(s)].nseg = S;
+    }
+    if (!tab.empty()) {
+        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->split_tables), tab.size() * sizeof(double), stream),
+                "hipMallocAsync (split lines)");
+        HIP_TRY(hipMemcpyAsync(b->split_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+        for (const auto& [s, off] : at) {
+            BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
+            const int64_t H = bs.ep.dim[0].n - 2 * bs.band[0] - 8;
+            bs.gf = b->split_tables + off;
+            bs.gb = bs.gf + H * bs.band[0];
+        }
+        HIP_TRY(hipMemcpyAsync(b->specs, b->host_specs.data(), b->host_specs.size() * sizeof(BatchSpline),
+                               hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->split_list), b->split_host.size() * sizeof(int32_t), stream),
+                "hipMallocAsync (split lines)");
+        HIP_TRY(hipMemcpyAsync(b->split_list, b->split_host.data(), b->split_host.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream),
+                "hipMemcpyAsync");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    b->split_ready = true;
+    return INTP_OK;
+}
+
+int launch_build(intp_batch b, const double* values, hipStream_t stream, bool split = false) {
     const bool lines = !b->list1d_host.empty();
     if (lines) {
         // fork: the 1D splines on the side stream, joined before returning
         HIP_TRY(hipEventRecord(b->fork, stream), "hipEventRecord");
         HIP_TRY(hipStreamWaitEvent(b->side, b->fork, 0), "hipStreamWaitEvent");
+        const bool seg = split && !b->split_host.empty();
+        if (seg) {
+            HIP_TRY(hipStreamWaitEvent(b->side2, b->fork, 0), "hipStreamWaitEvent");
+            hipLaunchKernelGGL(batch_line_split_kernel, dim3(static_cast<unsigned>(b->split_host.size())),
+                               dim3(kLineThreads), 0, b->side2, b->specs, b->split_list, values, b->ctrl);
+            HIP_TRY(hipGetLastError(), "batch split line launch");
+            HIP_TRY(hipEventRecord(b->join2, b->side2), "hipEventRecord");
+        }
         hipLaunchKernelGGL(batch_line_kernel, dim3(static_cast<unsigned>(b->list1d_host.size())), dim3(kLineThreads),
-                           0, b->side, b->specs, b->list1d, values, b->ctrl);
+                           0, b->side, b->specs, b->list1d, values, b->ctrl, seg ? 1 : 0);
         HIP_TRY(hipGetLastError(), "batch line launch");
         HIP_TRY(hipEventRecord(b->join, b->side), "hipEventRecord");
     }
@@ -429,6 +741,7 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream) {
         HIP_TRY(hipGetLastError(), "batch solve launch");
     }
     if (lines) HIP_TRY(hipStreamWaitEvent(stream, b->join, 0), "hipStreamWaitEvent");
+    if (lines && split && !b->split_host.empty()) HIP_TRY(hipStreamWaitEvent(stream, b->join2, 0), "hipStreamWaitEvent");
     return INTP_OK;
 }
 
@@ -473,16 +786,16 @@ void drop_graph(intp_batch b) {
 
 // Times one build + eval with the current tuning (median of 3, ms).
 float time_run(intp_batch b, const double* values, const double* coords, double* out, hipStream_t stream,
-               int* rc) {
+               int* rc, bool split) {
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     float t[3] = {0, 0, 0};
-    *rc = launch_build(b, values, stream);
+    *rc = launch_build(b, values, stream, split);
     if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);  // warm
     for (int r = 0; r < 3 && *rc == INTP_OK; ++r) {
         (void)hipEventRecord(e0, stream);
-        *rc = launch_build(b, values, stream);
+        *rc = launch_build(b, values, stream, split);
         if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);
         (void)hipEventRecord(e1, stream);
         (void)hipEventSynchronize(e1);
@@ -497,7 +810,7 @@ float time_run(intp_batch b, const double* values, const double* coords, double*
 // Coordinate descent over the tunables, each candidate timed on the
 // batch's own build + eval.
 int autotune(intp_batch b, const double* values, const double* coords, const int64_t* point_offsets, double* out,
-             hipStream_t stream) {
+             hipStream_t stream, bool split) {
     std::string report;
     auto try_values = [&](const char* name, int Tuning::*field, std::initializer_list<int> cands) -> int {
         int best = b->tune.*field;
@@ -508,7 +821,7 @@ int autotune(intp_batch b, const double* values, const double* coords, const int
             int rc = make_solve_work(b, stream);
             if (rc == INTP_OK) rc = make_eval_work(b, point_offsets, stream);
             if (rc != INTP_OK) return rc;
-            const float ms = time_run(b, values, coords, out, stream, &rc);
+            const float ms = time_run(b, values, coords, out, stream, &rc, split);
             if (rc != INTP_OK) return rc;
             line += " " + std::to_string(v) + "=" + std::to_string(ms).substr(0, 6) + "ms";
             if (ms < best_ms) {
@@ -696,6 +1009,8 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
         HIP_TRY(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
         HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
         HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
+        HIP_TRY(hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking), "hipStreamCreate");
+        HIP_TRY(hipEventCreateWithFlags(&b->join2, hipEventDisableTiming), "hipEventCreate");
     }
     const int rc = make_solve_work(b.get(), stream);
     if (rc != INTP_OK) return rc;
@@ -745,16 +1060,19 @@ int intp_batch_run(intp_batch b, const double* values, const double* coords, con
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     DeviceScope scope(b->device);
     int rc = INTP_OK;
+    const bool split = (flags & INTP_BATCH_SPLIT_LINES) != 0;
+    if (split && (rc = prepare_split(b, stream)) != INTP_OK) return rc;
     if ((flags & INTP_BATCH_AUTOTUNE) && !b->tuned)
-        if ((rc = autotune(b, values, coords, point_offsets, out, stream)) != INTP_OK) return rc;
+        if ((rc = autotune(b, values, coords, point_offsets, out, stream, split)) != INTP_OK) return rc;
     if ((rc = make_eval_work(b, point_offsets, stream)) != INTP_OK) return rc;
     if ((flags & INTP_BATCH_GRAPH) && stream != nullptr) {
         const void* key[4] = {values, coords, out, stream};
-        const bool same = b->exec && std::equal(key, key + 4, b->graph_key) && b->graph_layout == b->eval_layout;
+        const bool same = b->exec && std::equal(key, key + 4, b->graph_key) && b->graph_layout == b->eval_layout &&
+                          b->graph_split == split;
         if (!same) {
             drop_graph(b);
             HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            rc = launch_build(b, values, stream);
+            rc = launch_build(b, values, stream, split);
             if (rc == INTP_OK) rc = launch_eval_work(b, coords, out, stream);
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(stream, &g);
@@ -767,10 +1085,11 @@ int intp_batch_run(intp_batch b, const double* values, const double* coords, con
             HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             std::copy(key, key + 4, b->graph_key);
             b->graph_layout = b->eval_layout;
+            b->graph_split = split;
         }
         HIP_TRY(hipGraphLaunch(b->exec, stream), "hipGraphLaunch");
     } else {
-        rc = launch_build(b, values, stream);
+        rc = launch_build(b, values, stream, split);
         if (rc == INTP_OK) rc = launch_eval_work(b, coords, out, stream);
         if (rc != INTP_OK) return rc;
     }

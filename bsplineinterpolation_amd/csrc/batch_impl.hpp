@@ -25,6 +25,13 @@ struct BatchSpline {
     int32_t dim, order;
     int32_t lines_per_block[kMaxDim];  // LDS tile of the solve along each axis
     int32_t band[kMaxDim];             // band width of each axis's LU
+    // Split solve of a long 1D line (INTP_BATCH_SPLIT_LINES): nseg segments
+    // of the first n - 2*band - 8 rows, and the responses of each segment's
+    // rows to the band values entering it, forward (gf) and backward (gb),
+    // [row][band].  nseg == 0: the line is solved sequentially.
+    const double* gf;
+    const double* gb;
+    int32_t nseg;
 };
 
 // Work of one solve workgroup.  kind & 15 = 2 * band width + cyclic

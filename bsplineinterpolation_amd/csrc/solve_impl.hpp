@@ -189,12 +189,63 @@ constexpr int batch_rows() {
     return BW * (CYC ? 2 : 1) <= 4 ? kBatch : kBatch / 2;
 }
 
-// Solves one line x[0], x[step], ..., x[(n-1)*step] in place, computing
-// in C (elements of x have type P).  Software-pipelined: the loads of the next batch of rows (right-hand
-// sides and factor coefficients) are in flight while the current batch is
+// Forward substitution of rows a .. e-1 of a line from state `st`
+// (software-pipelined: the loads of the next batch of rows, right-hand
+// sides and factor coefficients, are in flight while the current batch is
 // substituted, so one lane's chain of dependent rows does not wait for
-// memory every batch.  Each row receives the same updates in the same
-// order as without pipelining.
+// memory every batch; each row receives the same updates in the same
+// order as without pipelining).
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void forward_range(P* __restrict__ x, int64_t step, int64_t n, int64_t a, int64_t e,
+                                              const SolveParams& sp, LineState<BW, CYC, C>& st, int32_t z) {
+    const int64_t nb = (e - a) / B;  // full batches
+    if (nb > 0) {
+        ForwardBatch<BW, CYC, C, B> ba, bb;
+        load_forward<BW, CYC, C, P, B>(ba, x, step, n, a, sp, z);
+        int64_t t = 0;
+        for (; t + 2 <= nb; t += 2) {
+            load_forward<BW, CYC, C, P, B>(bb, x, step, n, a + (t + 1) * B, sp, z);
+            run_forward<BW, CYC, C, P, B>(ba, x, step, n, a + t * B, st);
+            // batch t + 2, or (past the end) a harmless reload of batch nb - 1
+            load_forward<BW, CYC, C, P, B>(ba, x, step, n, a + min(t + 2, nb - 1) * B, sp, z);
+            run_forward<BW, CYC, C, P, B>(bb, x, step, n, a + (t + 1) * B, st);
+        }
+        if (t < nb) run_forward<BW, CYC, C, P, B>(ba, x, step, n, a + t * B, st);
+    }
+    for (int64_t j = a + nb * B; j < e; ++j) {
+        ForwardBatch<BW, CYC, C, 1> r;
+        load_forward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
+        run_forward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
+    }
+}
+
+// Backward substitution of rows e-1 down to a from state `st` (pipelined
+// like forward_range).
+template <int BW, bool CYC, typename C, typename P, int B>
+__device__ __forceinline__ void backward_range(P* __restrict__ x, int64_t step, int64_t n, int64_t a, int64_t e,
+                                               const SolveParams& sp, LineState<BW, CYC, C>& st, int32_t z) {
+    const int64_t nb = (e - a) / B;
+    if (nb > 0) {
+        BackwardBatch<BW, CYC, C, B> ba, bb;
+        load_backward<BW, CYC, C, P, B>(ba, x, step, n, e - 1, sp, z);
+        int64_t t = 0;
+        for (; t + 2 <= nb; t += 2) {
+            load_backward<BW, CYC, C, P, B>(bb, x, step, n, e - 1 - (t + 1) * B, sp, z);
+            run_backward<BW, CYC, C, P, B>(ba, x, step, n, e - 1 - t * B, st);
+            load_backward<BW, CYC, C, P, B>(ba, x, step, n, e - 1 - min(t + 2, nb - 1) * B, sp, z);
+            run_backward<BW, CYC, C, P, B>(bb, x, step, n, e - 1 - (t + 1) * B, st);
+        }
+        if (t < nb) run_backward<BW, CYC, C, P, B>(ba, x, step, n, e - 1 - t * B, st);
+    }
+    for (int64_t j = e - 1 - nb * B; j >= a; --j) {
+        BackwardBatch<BW, CYC, C, 1> r;
+        load_backward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
+        run_backward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
+    }
+}
+
+// Solves one line x[0], x[step], ..., x[(n-1)*step] in place, computing
+// in C (elements of x have type P).
 template <int BW, bool CYC, typename C, typename P, int B = batch_rows<BW, CYC>()>
 __device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int64_t n, const SolveParams& sp) {
     LineState<BW, CYC, C> st;
@@ -205,42 +256,8 @@ __device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int6
         for (int q = 0; q < BW; ++q) st.acc[q] = static_cast<C>(x[(n - BW + q) * step]);
     }
     const int32_t z = lane_zero();
-    const int64_t nb = n / B;  // full batches
-    if (nb > 0) {
-        ForwardBatch<BW, CYC, C, B> a, b;
-        load_forward<BW, CYC, C, P, B>(a, x, step, n, 0, sp, z);
-        int64_t t = 0;
-        for (; t + 2 <= nb; t += 2) {
-            load_forward<BW, CYC, C, P, B>(b, x, step, n, (t + 1) * B, sp, z);
-            run_forward<BW, CYC, C, P, B>(a, x, step, n, t * B, st);
-            // batch t + 2, or (past the end) a harmless reload of batch nb - 1
-            load_forward<BW, CYC, C, P, B>(a, x, step, n, min(t + 2, nb - 1) * B, sp, z);
-            run_forward<BW, CYC, C, P, B>(b, x, step, n, (t + 1) * B, st);
-        }
-        if (t < nb) run_forward<BW, CYC, C, P, B>(a, x, step, n, t * B, st);
-    }
-    for (int64_t j = nb * B; j < n; ++j) {
-        ForwardBatch<BW, CYC, C, 1> r;
-        load_forward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
-        run_forward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
-    }
-    if (nb > 0) {
-        BackwardBatch<BW, CYC, C, B> a, b;
-        load_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1, sp, z);
-        int64_t t = 0;
-        for (; t + 2 <= nb; t += 2) {
-            load_backward<BW, CYC, C, P, B>(b, x, step, n, n - 1 - (t + 1) * B, sp, z);
-            run_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - t * B, st);
-            load_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - min(t + 2, nb - 1) * B, sp, z);
-            run_backward<BW, CYC, C, P, B>(b, x, step, n, n - 1 - (t + 1) * B, st);
-        }
-        if (t < nb) run_backward<BW, CYC, C, P, B>(a, x, step, n, n - 1 - t * B, st);
-    }
-    for (int64_t j = n - 1 - nb * B; j >= 0; --j) {
-        BackwardBatch<BW, CYC, C, 1> r;
-        load_backward<BW, CYC, C, P, 1>(r, x, step, n, j, sp, z);
-        run_backward<BW, CYC, C, P, 1>(r, x, step, n, j, st);
-    }
+    forward_range<BW, CYC, C, P, B>(x, step, n, 0, n, sp, st, z);
+    backward_range<BW, CYC, C, P, B>(x, step, n, 0, n, sp, st, z);
 }
 
 // Fallback: one lane per line, substituting in HBM (storage T, compute C;

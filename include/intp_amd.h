@@ -203,6 +203,12 @@ typedef struct intp_batch_s* intp_batch;
                                   shapes (solve lines per tile, persistent
                                   workgroups per CU, points per eval
                                   workgroup) on this batch, keep the fastest */
+#define INTP_BATCH_SPLIT_LINES 4u /* solve 1D splines of >= 256 points as
+                                  up to 63 segments in parallel, chained by
+                                  precomputed segment responses: the same
+                                  linear system, but rounded differently
+                                  (within 1e-12 relative of the sequential
+                                  solve instead of bit-identical)          */
 int intp_batch_create(intp_ctx ctx, int32_t count, const intp_grid_desc* descs, intp_batch* out);
 int intp_batch_destroy(intp_batch b);
 /* Total values (= control points) and the per-spline offsets (count + 1). */

@@ -130,3 +130,43 @@ def test_gpu_batch_long_lines_and_singletons(amd):
         assert np.array_equal(b.control_points(s), g.control_points().ravel()), s
         D = len(sp["shape"])
         assert np.array_equal(out[po[s]:po[s + 1]], g.eval(pts[po[s]:po[s + 1], :D])[0]), s
+
+
+@pytest.mark.parametrize("order,periodic", [(2, False), (3, False), (5, False), (6, False), (7, False),
+                                            (2, True), (3, True), (5, True), (6, True)])
+def test_gpu_batch_split_lines(amd, orc, order, periodic):
+    """INTP_BATCH_SPLIT_LINES: long 1D lines solved in parallel segments
+    chained by precomputed responses.  Same linear system as the
+    sequential solve, rounded differently: control points and values within
+    1e-12 (norm-wise relative) of the sequential batch and of the oracle;
+    short lines and multi-dimensional splines stay bit-identical."""
+    import torch
+    rng = np.random.default_rng(order * 10 + periodic)
+    specs = [dict(order=order, shape=(4099,), periodic=[periodic], ranges=[(0.0, 1.0)]),
+             dict(order=order, shape=(1500,), periodic=[periodic], ranges=[(-1.0, 2.0)]),
+             dict(order=order, shape=(200,), periodic=[periodic], ranges=[(0.0, 1.0)]),
+             dict(order=3, shape=(40, 50), periodic=[periodic, False], ranges=[(0.0, 1.0), (0.0, 1.0)])]
+    b = amd.Batch(specs)
+    x = np.linspace(0.0, 1.0, b.total)
+    vals = torch.from_numpy(np.sin(40 * x) + 0.01 * rng.standard_normal(b.total)).cuda()
+    q = 500
+    pts = rng.random((q * len(specs), 3)) * 0.999
+    pts[q:2 * q, 0] = -1.0 + 3.0 * pts[q:2 * q, 0]
+    po = np.arange(len(specs) + 1) * q
+    pd = torch.from_numpy(pts).cuda()
+    seq = b.run(vals, pd, po).clone()
+    ctrl_seq = [b.control_points(s) for s in range(len(specs))]
+    split = b.run(vals, pd, po, split_lines=True, graph=True, stream=torch.cuda.Stream())
+    torch.cuda.synchronize()
+    for s in range(len(specs)):
+        c = b.control_points(s)
+        if s < 2:
+            assert rel_err(c, ctrl_seq[s]) < 1e-12, s
+            assert rel_err(split[po[s]:po[s + 1]].cpu().numpy(), seq[po[s]:po[s + 1]].cpu().numpy()) < 1e-12, s
+        else:  # below the split threshold, or not 1D: the sequential solve
+            assert np.array_equal(c, ctrl_seq[s]), s
+            assert torch.equal(split[po[s]:po[s + 1]], seq[po[s]:po[s + 1]]), s
+    f0 = vals[b.offsets[0]:b.offsets[1]].cpu().numpy()
+    o = orc.OracleSpline(order, f0, periodic=[periodic], ranges=[(0.0, 1.0)])
+    want = o.eval(np.ascontiguousarray(pts[:q, :1]))[0]
+    assert rel_err(split[:q].cpu().numpy(), want) < 1e-12
