@@ -7,12 +7,12 @@
 // an earlier turn):
 //   1. take every queued request, oldest first;
 //      eval  -> launched on the next stream, after a stream wait on its
-//               spline's build event; fails with INTP_E_EVICTED if the
-//               spline is gone;
+//               spline's build event; waits while its build is queued;
+//               fails with INTP_E_EVICTED if the spline is gone;
 //      build -> admitted if its control points fit the budget beside the
 //               resident splines, after evicting least-recently-used idle
 //               splines if needed; otherwise it stays queued for a later
-//               turn (evaluations behind it are not held up);
+//               turn (evaluations of other splines are not held up);
 //   2. poll the events of launched requests; a finished request gets its
 //      completion time, unpins its spline, and wakes waiting clients;
 //   3. sleep until a submission, or briefly while work is in flight.
@@ -200,8 +200,9 @@ void intp_server_s::drop(std::unordered_map<int64_t, Resident>::iterator it) {
     splines.erase(it);
 }
 
-// Admits and launches one request.  Returns 1 if it ran or failed, 0 if a
-// build must wait for memory, -1 if the in-flight limit is reached.
+// Admits and launches one request.  Returns 1 if it ran or failed, 0 if it
+// must wait (a build for memory, an evaluation for its queued build), -1
+// if the in-flight limit is reached.
 // Called and returns with lk held; drops it around the HIP launches.
 int intp_server_s::dispatch(int64_t id, std::unique_lock<std::mutex>& lk) {
     if (static_cast<int>(inflight.size()) >= max_inflight) return -1;
@@ -263,6 +264,12 @@ int intp_server_s::dispatch(int64_t id, std::unique_lock<std::mutex>& lk) {
         r.done = ev;
     } else {
         auto it = splines.find(r.spline);
+        if (it == splines.end()) {
+            // its build may still be queued, waiting for memory: wait with it
+            auto bq = reqs.find(r.spline);
+            if (bq != reqs.end() && bq->second->kind == INTP_REQ_BUILD && bq->second->state == INTP_REQ_QUEUED)
+                return 0;
+        }
         if (it == splines.end() || it->second.released) {
             finish(r, fail(INTP_E_EVICTED, "spline " + std::to_string(r.spline) + " is not resident"));
             return 1;
@@ -334,10 +341,10 @@ void intp_server_s::poll_inflight() {
 void intp_server_s::run() {
     DeviceScope scope(device);
     std::unique_lock<std::mutex> lk(mu);
-    std::deque<int64_t> waiting;  // builds held back for memory, oldest first
+    std::deque<int64_t> waiting;  // requests held back (memory, their build), oldest first
     for (;;) {
         bool progressed = false;
-        // builds that waited for memory go first, in arrival order
+        // requests that waited go first, in arrival order
         std::deque<int64_t> turn;
         turn.swap(waiting);
         while (!queue.empty()) {

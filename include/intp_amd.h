@@ -233,7 +233,8 @@ int intp_batch_download_ctrl(intp_batch b, int32_t spline, double* host, int64_t
  *   build: admitted when its control points fit the HBM budget next to
  *     the resident splines; least-recently-used splines with no pending
  *     evaluation are evicted to make room; a build that fits only after
- *     running work completes waits in the queue (evaluations pass it);
+ *     running work completes waits in the queue (evaluations of other
+ *     splines pass it, its own wait with it);
  *   eval: runs on the spline's table once its build has finished (stream
  *     ordered, so an eval may be submitted right after its build).
  * Results are those of intp_build / intp_build_refined / intp_eval on the

@@ -193,3 +193,42 @@ def test_gpu_serve_concurrent_clients_and_nonuniform_grids(amd):
         for r in res:
             assert np.array_equal(r, direct), i
     srv.close()
+
+
+def test_gpu_serve_memory_pressure(amd):
+    """A budget of three splines and a burst of builds with evaluations
+    queued behind them (no waits in between): builds wait for memory while
+    pinned splines hold it, idle ones are evicted, and every request ends
+    either done (with the direct call's result) or failed as evicted."""
+    import torch
+    rng = np.random.default_rng(17)
+    shape = (48, 40)
+    one = 48 * 40 * 8
+    srv = amd.Server(hbm_budget=3 * one, streams=3, max_inflight=8)
+    fs = [torch.from_numpy(rng.standard_normal(shape)).cuda() for _ in range(12)]
+    pts = torch.from_numpy(np.column_stack([rng.random(4000), rng.random(4000)])).cuda()
+    builds, evals = [], []
+    for k, f in enumerate(fs):
+        sid = srv.build(3, f, ranges=[(0, 1), (0, 1)])
+        builds.append(sid)
+        for back in range(3):  # this spline and the two before it
+            if k - back >= 0:
+                rid = srv.eval(builds[k - back], pts, [(0, 0), (1, 0)])
+                evals.append((k - back, rid, srv._hold[rid][1]))
+    srv.drain()
+    st = srv.stats()
+    assert st["build"]["completed"] == 12 and st["build"]["failed"] == 0
+    assert st["resident_bytes"] <= 3 * one
+    done = 0
+    for k, rid, out in evals:
+        state, status = srv.poll(rid)
+        assert state in (amd._lib.INTP_REQ_DONE, amd._lib.INTP_REQ_FAILED)
+        if state == amd._lib.INTP_REQ_DONE:
+            g = amd.InterpolationFunction(3, fs[k], ranges=[(0, 1), (0, 1)])
+            assert torch.equal(out, g.eval(pts, orders=[(0, 0), (1, 0)])), k
+            done += 1
+        else:
+            assert status == amd._lib.INTP_E_EVICTED
+    # each build's own evaluation is queued right behind it, so it runs
+    assert done >= 12
+    srv.close()
