@@ -182,6 +182,34 @@ hipError_t launch_shift_copy(const ShiftParams& p, bool dst_f32, hipStream_t s);
 // y = M x along one axis of a table viewed as (outer, n, inner), M the
 // axis' collocation matrix as rows colloc[j*(2w+1) + (m+w)] = M(j, j+m)
 // (columns mod n if periodic); with src set, y = (shifted src) - M x.
+// Division of 32-bit unsigned integers by an invariant d >= 1 with a
+// multiply-high (Granlund & Montgomery, "Division by invariant integers
+// using multiplication", PLDI 1994, Fig. 4.1): for s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1, q = (t + ((x - t) >> 1)) >> (s - 1)
+// with t = mulhi(x, m); d == 1 passes x through.
+struct FastDiv {
+    uint32_t d = 1, m = 0, s = 0;
+};
+
+inline FastDiv make_fast_div(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    if (d <= 1) return f;
+    uint32_t s = 0;
+    while ((uint64_t{1} << s) < d) ++s;
+    f.s = s;
+    f.m = static_cast<uint32_t>(((uint64_t{1} << 32) * ((uint64_t{1} << s) - d)) / d + 1);
+    return f;
+}
+
+#ifdef __HIPCC__
+__device__ __forceinline__ uint32_t fast_div(uint32_t x, const FastDiv& f) {
+    if (f.d == 1) return x;
+    const uint32_t t = __umulhi(x, f.m);
+    return (t + ((x - t) >> 1)) >> (f.s - 1);
+}
+#endif
+
 struct MatvecParams {
     const void* x;  // float or double
     void* y;        // float or double
@@ -192,6 +220,7 @@ struct MatvecParams {
     int32_t dim;
     int64_t dn[kMaxDim];
     int64_t dshift[kMaxDim];
+    FastDiv div_inner, div_n, div_dn[kMaxDim];  // set by launch_band_matvec
 };
 hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s);
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s);

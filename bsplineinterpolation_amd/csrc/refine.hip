@@ -21,32 +21,39 @@ constexpr int kBlock = 256;
 
 // One output element per thread.  The table is viewed as (outer, n, inner)
 // along the applied axis; indices use 32-bit arithmetic (tables below
-// 2^32 elements).
-template <typename TI, typename TO>
+// 2^32 elements) and multiply-high divisions by the invariant extents.
+// The half-width W is a template argument so that the 2W+1 loads of a
+// row are all issued before the first product.
+template <int W, typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void band_matvec_kernel(MatvecParams p) {
     const uint64_t e64 = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (e64 >= static_cast<uint64_t>(p.total)) return;
     const uint32_t e = static_cast<uint32_t>(e64);
     const uint32_t n = static_cast<uint32_t>(p.n), inner = static_cast<uint32_t>(p.inner);
-    const uint32_t t = e / inner, ii = e - t * inner;
-    const uint32_t j = t % n;
+    const uint32_t t = fast_div(e, p.div_inner);
+    const uint32_t j = t - fast_div(t, p.div_n) * n;
     const uint32_t base = e - j * inner;
     const TI* __restrict__ x = static_cast<const TI*>(p.x);
-    const int w = p.w;
-    const double* __restrict__ row = p.colloc + static_cast<uint32_t>(j) * static_cast<uint32_t>(2 * w + 1);
-    double acc = 0.0;
-    for (int m = -w; m <= w; ++m) {
+    const double* __restrict__ row = p.colloc + j * static_cast<uint32_t>(2 * W + 1);
+    // out-of-range columns of a clamped dimension read element j and count
+    // as zero (their matrix entries are zero too)
+    double xv[2 * W + 1];
+#pragma unroll
+    for (int m = -W; m <= W; ++m) {
         int64_t jj = static_cast<int64_t>(j) + m;
+        bool ok = true;
         if (p.periodic) {
-            if (jj < 0)
-                jj += n;
-            else if (jj >= n)
-                jj -= n;
-        } else if (jj < 0 || jj >= static_cast<int64_t>(n)) {
-            continue;
+            jj = jj < 0 ? jj + n : jj >= n ? jj - n : jj;
+        } else {
+            ok = jj >= 0 && jj < n;
+            jj = ok ? jj : j;
         }
-        acc += row[m + w] * static_cast<double>(x[base + static_cast<uint32_t>(jj) * inner]);
+        const double v = static_cast<double>(x[base + static_cast<uint32_t>(jj) * inner]);
+        xv[m + W] = ok ? v : 0.0;
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m <= 2 * W; ++m) acc += row[m] * xv[m];
     if (p.src) {
         // residual: the shifted value minus the product; element
         // (i_0..i_{D-1}) reads source element ((i_d - shift_d) mod n_d)
@@ -54,8 +61,9 @@ __global__ __launch_bounds__(kBlock) void band_matvec_kernel(MatvecParams p) {
         uint64_t src = 0, mul = 1;
         for (int d = p.dim - 1; d >= 0; --d) {
             const uint32_t nd = static_cast<uint32_t>(p.dn[d]);
-            const uint32_t id = rem % nd;
-            rem /= nd;
+            const uint32_t qd = fast_div(rem, p.div_dn[d]);
+            const uint32_t id = rem - qd * nd;
+            rem = qd;
             int64_t s = static_cast<int64_t>(id) - p.dshift[d];
             if (s < 0) s += nd;
             src += static_cast<uint64_t>(s) * mul;
@@ -72,22 +80,40 @@ __global__ __launch_bounds__(kBlock) void add_f32_kernel(float* __restrict__ c, 
     if (e < total) c[e] = c[e] + d[e];
 }
 
-}  // namespace
-
-hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s) {
-    if (p.total >= (int64_t{1} << 32)) return hipErrorInvalidValue;
-    const int64_t blocks = (p.total + kBlock - 1) / kBlock;
-    if (blocks == 0) return hipSuccess;
+template <int W>
+hipError_t launch_matvec_width(const MatvecParams& p, bool in_f32, bool out_f32, int64_t blocks, hipStream_t s) {
     const dim3 grid(static_cast<unsigned>(blocks)), block(kBlock);
     if (in_f32 && out_f32)
-        hipLaunchKernelGGL((band_matvec_kernel<float, float>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((band_matvec_kernel<W, float, float>), grid, block, 0, s, p);
     else if (in_f32)
-        hipLaunchKernelGGL((band_matvec_kernel<float, double>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((band_matvec_kernel<W, float, double>), grid, block, 0, s, p);
     else if (out_f32)
-        hipLaunchKernelGGL((band_matvec_kernel<double, float>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((band_matvec_kernel<W, double, float>), grid, block, 0, s, p);
     else
-        hipLaunchKernelGGL((band_matvec_kernel<double, double>), grid, block, 0, s, p);
+        hipLaunchKernelGGL((band_matvec_kernel<W, double, double>), grid, block, 0, s, p);
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_band_matvec(const MatvecParams& p_in, bool in_f32, bool out_f32, hipStream_t s) {
+    if (p_in.total >= (int64_t{1} << 32)) return hipErrorInvalidValue;
+    MatvecParams p = p_in;
+    p.div_inner = make_fast_div(static_cast<uint32_t>(p.inner));
+    p.div_n = make_fast_div(static_cast<uint32_t>(p.n));
+    for (int d = 0; d < p.dim && d < kMaxDim; ++d) p.div_dn[d] = make_fast_div(static_cast<uint32_t>(p.dn[d]));
+    const int64_t blocks = (p.total + kBlock - 1) / kBlock;
+    if (blocks == 0) return hipSuccess;
+    switch (p.w) {
+        case 0: return launch_matvec_width<0>(p, in_f32, out_f32, blocks, s);
+        case 1: return launch_matvec_width<1>(p, in_f32, out_f32, blocks, s);
+        case 2: return launch_matvec_width<2>(p, in_f32, out_f32, blocks, s);
+        case 3: return launch_matvec_width<3>(p, in_f32, out_f32, blocks, s);
+        case 4: return launch_matvec_width<4>(p, in_f32, out_f32, blocks, s);
+        case 5: return launch_matvec_width<5>(p, in_f32, out_f32, blocks, s);
+        case 6: return launch_matvec_width<6>(p, in_f32, out_f32, blocks, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s) {
