@@ -60,6 +60,24 @@ __device__ __forceinline__ void load_point(const BinParams& p, int64_t i, double
     for (int d = 0; d < kMaxDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
 }
 
+// A record read once, as two 16-byte non-temporal loads.  Streamed
+// through L2 at the default priority, the 2 GB of input records evict the
+// partly written lines of the digit runs before their neighbouring runs
+// complete them: the L2 then writes each output line in pieces (1.8x the
+// record bytes in 64-byte write requests) and fetches input lines twice.
+__device__ __forceinline__ BinRecord load_record_stream(const BinRecord* src) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(BinRecord) == 2 * sizeof(u32x4), "BinRecord is two 16-byte halves");
+    const u32x4* s = reinterpret_cast<const u32x4*>(src);
+    union {
+        u32x4 v[2];
+        BinRecord r;
+    } u;
+    u.v[0] = __builtin_nontemporal_load(s);
+    u.v[1] = __builtin_nontemporal_load(s + 1);
+    return u.r;
+}
+
 // ---- exclusive scan of n uint32 (n <= kScanTile * kScanTile) ------------
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
@@ -138,7 +156,7 @@ __global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, cons
                 r[j].idx = static_cast<uint32_t>(i);
                 r[j].key = brick_key(p, r[j].x);
             } else {
-                r[j] = p.rec_in[i];
+                r[j] = load_record_stream(p.rec_in + i);
             }
             digit[j] = PASS == 0 ? (r[j].key & 0xFFu) : ((r[j].key >> 8) & 0xFFu);
             rank[j] = atomicAdd(&cnt[digit[j]], 1u);
