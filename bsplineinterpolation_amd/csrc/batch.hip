@@ -72,7 +72,9 @@ __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const
         if (first_pass) {
             sp.src = values + b.offset;
             sp.outer_dims = a;
-            for (int d = 0; d < a; ++d) {
+#pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) {
+                if (d >= a) break;
                 sp.outer_n[d] = b.ep.dim[d].n;
                 sp.outer_shift[d] = b.shift[d];
             }

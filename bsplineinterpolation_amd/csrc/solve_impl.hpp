@@ -286,8 +286,12 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
         dst_base[tid] = o * n * inner + ii;
         if (sp.src) {
             // inner == 1: o runs over the outer dims in C order
+            // constant indices (a runtime-indexed loop would force a
+            // SolveParams built in registers out to scratch memory)
             int64_t rem = o, off = 0, mul = n;
-            for (int d = sp.outer_dims - 1; d >= 0; --d) {
+#pragma unroll
+            for (int d = kMaxDim - 1; d >= 0; --d) {
+                if (d >= sp.outer_dims) continue;
                 const int64_t nd = sp.outer_n[d];
                 const int64_t id = rem % nd;
                 rem /= nd;
