@@ -326,9 +326,11 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
                 from[u] = src_base[l] + js;
                 at[u] = l * static_cast<uint32_t>(stride) + jj;
             }
+            // the values are read once: non-temporal loads keep them from
+            // evicting the factors and the lines being written
             C v[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(sp.src[from[u]]);
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(__builtin_nontemporal_load(&sp.src[from[u]]));
 #pragma unroll
             for (int u = 0; u < kBatch; ++u)
                 if (e0 + u * kTileThreads < count) tile[at[u]] = v[u];
