@@ -18,8 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import DomainError, IntpError, check  # noqa: F401
 
-__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "Batch", "DomainError", "IntpError",
-           "eval_timings"]
+__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "Batch", "Server", "DomainError",
+           "IntpError", "eval_timings"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -60,7 +60,7 @@ class Context:
         return cls._defaults[device]
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
             _lib._LIB.intp_ctx_destroy(self.h)
             self.h = None
 
@@ -163,7 +163,7 @@ class InterpolationFunction:
         return cls._wrap(h, ctx)
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
             _lib._LIB.intp_spline_release(self.h)
             self.h = None
 
@@ -302,7 +302,7 @@ class InterpolationFunctionTemplate:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
             _lib._LIB.intp_template_destroy(self.h)
             self.h = None
 
@@ -355,7 +355,7 @@ class Batch:
         self.total = int(total.value)
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
             _lib._LIB.intp_batch_destroy(self.h)
             self.h = None
 
@@ -427,7 +427,7 @@ class Server:
         self._hold = {}
 
     def close(self):
-        if getattr(self, "h", None) is not None and _lib._LIB is not None:
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
             _lib._LIB.intp_server_destroy(self.h)
             self.h = None
             self._hold.clear()
