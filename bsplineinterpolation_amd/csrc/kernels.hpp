@@ -221,6 +221,7 @@ struct MatvecParams {
     int64_t dn[kMaxDim];
     int64_t dshift[kMaxDim];
     FastDiv div_inner, div_n, div_dn[kMaxDim];  // set by launch_band_matvec
+    int32_t xcd_remap;                            // set by launch_band_matvec
 };
 hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s);
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s);

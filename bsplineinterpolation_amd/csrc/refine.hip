@@ -24,9 +24,13 @@ constexpr int kBlock = 256;
 // 2^32 elements) and multiply-high divisions by the invariant extents.
 // The half-width W is a template argument so that the 2W+1 loads of a
 // row are all issued before the first product.
+// With xcd_remap, each XCD takes a contiguous range of blocks, so that
+// the blocks of neighbouring rows (which share 2W of their 2W+1 input
+// rows) meet in one L2; otherwise block b goes to XCD b mod 8.
 template <int W, typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void band_matvec_kernel(MatvecParams p) {
-    const uint64_t e64 = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t blk = p.xcd_remap ? static_cast<uint64_t>(xcd_chunk_index()) : blockIdx.x;
+    const uint64_t e64 = blk * kBlock + threadIdx.x;
     if (e64 >= static_cast<uint64_t>(p.total)) return;
     const uint32_t e = static_cast<uint32_t>(e64);
     const uint32_t n = static_cast<uint32_t>(p.n), inner = static_cast<uint32_t>(p.inner);
@@ -102,6 +106,10 @@ hipError_t launch_band_matvec(const MatvecParams& p_in, bool in_f32, bool out_f3
     p.div_inner = make_fast_div(static_cast<uint32_t>(p.inner));
     p.div_n = make_fast_div(static_cast<uint32_t>(p.n));
     for (int d = 0; d < p.dim && d < kMaxDim; ++d) p.div_dn[d] = make_fast_div(static_cast<uint32_t>(p.dn[d]));
+    // the blocks of rows j and j+1 are inner / kBlock apart: under the
+    // default round-robin over 8 XCDs they share an L2 only when that
+    // distance is a multiple of 8
+    p.xcd_remap = p.inner > 1 && (p.inner % kBlock != 0 || (p.inner / kBlock) % 8 != 0) ? 1 : 0;
     const int64_t blocks = (p.total + kBlock - 1) / kBlock;
     if (blocks == 0) return hipSuccess;
     switch (p.w) {
