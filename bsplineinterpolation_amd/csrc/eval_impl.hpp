@@ -511,6 +511,26 @@ __global__ __launch_bounds__(kEvalBlock) void eval_uniform_kernel(EvalParams p) 
 
 // ---- grouped evaluation with the brick patch in LDS ---------------------
 
+// One grouped record as two 16-byte loads (the patch kernels read the
+// records of the grouping passes: p.coords points at them).
+constexpr bool kRecordStream = false;
+__device__ __forceinline__ BinRecord load_record(const BinRecord* src) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* s = reinterpret_cast<const u32x4*>(src);
+    union {
+        u32x4 v[2];
+        BinRecord r;
+    } u;
+    if constexpr (kRecordStream) {
+        u.v[0] = __builtin_nontemporal_load(s);
+        u.v[1] = __builtin_nontemporal_load(s + 1);
+    } else {
+        u.v[0] = s[0];
+        u.v[1] = s[1];
+    }
+    return u.r;
+}
+
 template <typename T>
 constexpr int patch_max() {
     return kPatchBytes / static_cast<int>(sizeof(T));
@@ -524,7 +544,9 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
                                             const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
                                             const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
                                             const uint32_t (&lstride)[D], int64_t pt) {
-    const int64_t opt = static_cast<int64_t>(p.perm[pt * p.perm_stride]);
+    // the grouped record (coordinates, caller position): two 16-byte loads
+    const BinRecord rec = load_record(reinterpret_cast<const BinRecord*>(p.coords) + pt);
+    const int64_t opt = static_cast<int64_t>(rec.idx);
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
     DimW<K, uint32_t, T> dd[D];
     bool zero = false, inside = true;
@@ -533,7 +555,7 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r,
+        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], rec.x[d], r,
                                                    p.bad, dd[d]);
         const int n = static_cast<int>(p.dim[d].n);
         int st = mu - K;  // first stencil point; periodic even orders: may be n (wraps to 0)
@@ -686,7 +708,10 @@ hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
 template <int D, int K, bool LASTC, typename T>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
     if (uniform && p.patch_volume > 0) {
-        if (D < 2 || p.patch_volume > patch_max<T>() || !p.perm || !p.key) return hipErrorInvalidValue;
+        // the patch kernels read the grouping passes' records
+        if (D < 2 || p.patch_volume > patch_max<T>() || !p.perm || !p.key || p.cs_dim != 1 ||
+            p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
+            return hipErrorInvalidValue;
         if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC, T>>(p, s);
         return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
     }
