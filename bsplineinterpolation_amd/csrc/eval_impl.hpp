@@ -201,10 +201,10 @@ __device__ __forceinline__ int uniform_span(const DimParams& dp, double x) {
 }
 
 // Weights from the span's basis polynomials P (row j: coefficients of
-// basis j in u = (x - t_mu) / dx, ascending powers).
-template <int K, bool GRAD, typename W>
-__device__ __forceinline__ void poly_weights(const double* __restrict__ P, double u, double inv_dx, int r,
-                                             DimW<K, uint32_t, W>& dd) {
+// basis j in u = (x - t_mu) / dx, ascending powers); PP is a pointer to
+// the coefficients in a given address space.
+template <int K, bool GRAD, typename W, typename PP>
+__device__ __forceinline__ void poly_weights(PP P, double u, double inv_dx, int r, DimW<K, uint32_t, W>& dd) {
     constexpr int NW = K + 1;
     if constexpr (GRAD) {
 #pragma unroll
@@ -263,10 +263,14 @@ __device__ __forceinline__ int uniform_weights(const DimParams& dp, const double
     const int mu = uniform_span<K>(dp, x);
     const int cls = (mu - K < dp.nleft) ? 1 + (mu - K) : ((last - mu < dp.nright) ? 1 + dp.nleft + (last - mu) : 0);
     const double u = (x - uknot_sel<K>(dp, mu)) * dp.inv_dx;
+    // explicit address spaces: merged into one generic pointer, both
+    // branches would read through flat loads (LDS at memory latency, and
+    // no scalar loads for the uniform case)
     if (__all(cls == 0))
-        poly_weights<K, GRAD, W>(dp.poly, u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W>((const __attribute__((address_space(4))) double*)dp.poly, u, dp.inv_dx, r, dd);
     else
-        poly_weights<K, GRAD, W>(poly + cls * (K + 1) * (K + 1), u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W>((const __attribute__((address_space(3))) double*)(poly + cls * (K + 1) * (K + 1)), u,
+                                 dp.inv_dx, r, dd);
     stencil_offsets<K>(dp, mu, dd.off);
     return mu;
 }
