@@ -61,6 +61,18 @@ __device__ __forceinline__ int32_t lane_zero() {
     return z;
 }
 
+// Factor coefficients live in global memory.  Pointers read from a
+// spline record (the batched kernels) are generic, and generic loads are
+// flat loads, which count on the LDS counter as well: the same waits as
+// scalar loads.  The explicit address space keeps them global loads.
+template <typename C>
+using GlobalPtr = const __attribute__((address_space(1))) C*;
+
+template <typename C>
+__device__ __forceinline__ GlobalPtr<C> factor_ptr(const void* p, int32_t z) {
+    return (GlobalPtr<C>)(p) + z;
+}
+
 // Registers of one batch of forward rows: the loaded right-hand sides and
 // the factor coefficients of those rows.
 template <int BW, bool CYC, typename C, int B>
@@ -82,8 +94,8 @@ struct BackwardBatch {
 template <int BW, bool CYC, typename C, typename P, int B>
 __device__ __forceinline__ void load_forward(ForwardBatch<BW, CYC, C, B>& r, const P* __restrict__ x, int64_t step,
                                              int64_t n, int64_t j0, const SolveParams& sp, int32_t z) {
-    const C* __restrict__ lower = static_cast<const C*>(sp.lower) + z;
-    const C* __restrict__ bottom = static_cast<const C*>(sp.bottom) + z;
+    const GlobalPtr<C> __restrict__ lower = factor_ptr<C>(sp.lower, z);
+    const GlobalPtr<C> __restrict__ bottom = factor_ptr<C>(sp.bottom, z);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         r.xb[u] = static_cast<C>(x[(j0 + u) * step]);
@@ -131,9 +143,9 @@ __device__ __forceinline__ void run_forward(ForwardBatch<BW, CYC, C, B>& r, P* _
 template <int BW, bool CYC, typename C, typename P, int B>
 __device__ __forceinline__ void load_backward(BackwardBatch<BW, CYC, C, B>& r, const P* __restrict__ x, int64_t step,
                                               int64_t n, int64_t j0, const SolveParams& sp, int32_t z) {
-    const C* __restrict__ upper = static_cast<const C*>(sp.upper) + z;
-    const C* __restrict__ diag = static_cast<const C*>(sp.diag) + z;
-    const C* __restrict__ right = static_cast<const C*>(sp.right) + z;
+    const GlobalPtr<C> __restrict__ upper = factor_ptr<C>(sp.upper, z);
+    const GlobalPtr<C> __restrict__ diag = factor_ptr<C>(sp.diag, z);
+    const GlobalPtr<C> __restrict__ right = factor_ptr<C>(sp.right, z);
 #pragma unroll
     for (int u = 0; u < B; ++u) {
         const int64_t j = j0 - u;
