@@ -786,16 +786,19 @@ void drop_graph(intp_batch b) {
     b->graph = nullptr;
 }
 
-// Times one build + eval with the current tuning (median of 3, ms).
+// Times one build + eval with the current tuning (the fastest of 5, ms:
+// the candidates differ by a few percent, less than run-to-run noise of a
+// median of 3).
 float time_run(intp_batch b, const double* values, const double* coords, double* out, hipStream_t stream,
                int* rc, bool split) {
+    constexpr int kRuns = 5;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    float t[3] = {0, 0, 0};
+    float t[kRuns] = {};
     *rc = launch_build(b, values, stream, split);
     if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);  // warm
-    for (int r = 0; r < 3 && *rc == INTP_OK; ++r) {
+    for (int r = 0; r < kRuns && *rc == INTP_OK; ++r) {
         (void)hipEventRecord(e0, stream);
         *rc = launch_build(b, values, stream, split);
         if (*rc == INTP_OK) *rc = launch_eval_work(b, coords, out, stream);
@@ -805,8 +808,7 @@ float time_run(intp_batch b, const double* values, const double* coords, double*
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    std::sort(t, t + 3);
-    return t[1];
+    return *std::min_element(t, t + kRuns);
 }
 
 // Coordinate descent over the tunables, each candidate timed on the
@@ -835,11 +837,17 @@ int autotune(intp_batch b, const double* values, const double* coords, const int
         report += line + " -> " + std::to_string(best) + "; ";
         return INTP_OK;
     };
+    // the solve knobs dominate and interact: they are visited twice
     int rc;
-    if ((rc = try_values("eval_threads", &Tuning::eval_threads, {64, 128, 256})) != INTP_OK) return rc;
-    if ((rc = try_values("eval_blocks_per_cu", &Tuning::eval_blocks_per_cu, {2, 4, 8, 16})) != INTP_OK) return rc;
-    if ((rc = try_values("tile_cap", &Tuning::tile_cap, {8, 16, 32, 64})) != INTP_OK) return rc;
-    if ((rc = try_values("solve_blocks_per_cu", &Tuning::solve_blocks_per_cu, {1, 2, 4, 8})) != INTP_OK) return rc;
+    for (int round = 0; round < 2; ++round) {
+        if ((rc = try_values("tile_cap", &Tuning::tile_cap, {8, 16, 32, 64})) != INTP_OK) return rc;
+        if ((rc = try_values("solve_blocks_per_cu", &Tuning::solve_blocks_per_cu, {1, 2, 4, 8})) != INTP_OK) return rc;
+        if (round == 0) {
+            if ((rc = try_values("eval_threads", &Tuning::eval_threads, {64, 128, 256})) != INTP_OK) return rc;
+            if ((rc = try_values("eval_blocks_per_cu", &Tuning::eval_blocks_per_cu, {2, 4, 8, 16})) != INTP_OK)
+                return rc;
+        }
+    }
     if ((rc = make_solve_work(b, stream)) != INTP_OK) return rc;
     if ((rc = make_eval_work(b, point_offsets, stream)) != INTP_OK) return rc;
     b->tuned = true;
