@@ -179,9 +179,10 @@ __device__ __forceinline__ void batch_line_split(const BatchSpline& b, const dou
     sp.diag = b.diag[0];
     sp.bottom = b.bottom[0];
     sp.right = b.right[0];
-    const double* __restrict__ bottom = b.bottom[0];
-    const double* __restrict__ gf = b.gf;
-    const double* __restrict__ gb = b.gb;
+    // global pointers (generic ones read from the record would be flat loads)
+    const GlobalPtr<double> __restrict__ bottom = (GlobalPtr<double>)b.bottom[0];
+    const GlobalPtr<double> __restrict__ gf = (GlobalPtr<double>)b.gf;
+    const GlobalPtr<double> __restrict__ gb = (GlobalPtr<double>)b.gb;
     const int32_t z = lane_zero();
     // forward, zero state entering each segment (the head has no strip rows)
     if (lane < S) {
