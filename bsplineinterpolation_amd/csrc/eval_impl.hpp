@@ -517,7 +517,6 @@ __global__ __launch_bounds__(kEvalBlock) void eval_uniform_kernel(EvalParams p) 
 
 // One grouped record as two 16-byte loads (the patch kernels read the
 // records of the grouping passes: p.coords points at them).
-constexpr bool kRecordStream = false;
 __device__ __forceinline__ BinRecord load_record(const BinRecord* src) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const u32x4* s = reinterpret_cast<const u32x4*>(src);
@@ -525,13 +524,8 @@ __device__ __forceinline__ BinRecord load_record(const BinRecord* src) {
         u32x4 v[2];
         BinRecord r;
     } u;
-    if constexpr (kRecordStream) {
-        u.v[0] = __builtin_nontemporal_load(s);
-        u.v[1] = __builtin_nontemporal_load(s + 1);
-    } else {
-        u.v[0] = s[0];
-        u.v[1] = s[1];
-    }
+    u.v[0] = s[0];
+    u.v[1] = s[1];
     return u.r;
 }
 
@@ -547,9 +541,7 @@ template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void patch_point(const EvalParams& p,
                                             const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
                                             const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
-                                            const uint32_t (&lstride)[D], int64_t pt) {
-    // the grouped record (coordinates, caller position): two 16-byte loads
-    const BinRecord rec = load_record(reinterpret_cast<const BinRecord*>(p.coords) + pt);
+                                            const uint32_t (&lstride)[D], const BinRecord& rec) {
     const int64_t opt = static_cast<int64_t>(rec.idx);
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
     DimW<K, uint32_t, T> dd[D];
@@ -610,6 +602,7 @@ __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
     }
     const int64_t first = block_index() * kEvalBlock;
     const int64_t pivot = min(first + kEvalBlock / 2, p.npts - 1);
+    const int64_t pt = first + threadIdx.x;
     uint32_t rem = p.key[pivot * p.perm_stride];
     int bstart[D], bsize[D], psize[D];
     uint32_t lstride[D];
@@ -696,8 +689,11 @@ __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
         // stencil of a point inside the brick reaches them
     }
     __syncthreads();
-    const int64_t pt = first + threadIdx.x;
-    if (pt < p.npts) patch_point<D, K, GRAD, LASTC, T>(p, s_poly, s_patch, bstart, bsize, lstride, pt);
+    // the thread's grouped record (coordinates, caller position); loaded
+    // after the barrier (issued before the staging, it measured slower)
+    if (pt < p.npts)
+        patch_point<D, K, GRAD, LASTC, T>(p, s_poly, s_patch, bstart, bsize, lstride,
+                                          load_record(reinterpret_cast<const BinRecord*>(p.coords) + pt));
 }
 
 template <void (*KF)(EvalParams)>
