@@ -78,9 +78,7 @@ namespace {
 // item's spline parameters are copied into LDS, then one thread per point.
 template <int D, int K>
 __global__ __launch_bounds__(256) void batch_eval_kernel(BatchEvalArgs a) {
-    constexpr int kCoef = (K + 1) * (K + 1);
     __shared__ EvalParams sp;
-    __shared__ double s_poly[D][(1 + 2 * K) * kCoef];
     for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const BatchEvalItem w = a.items[it];
         const BatchSpline* __restrict__ b = a.specs + w.spline;
@@ -96,22 +94,15 @@ __global__ __launch_bounds__(256) void batch_eval_kernel(BatchEvalArgs a) {
         }
         const bool uniform = sp.dim[0].uniform && (D < 2 || sp.dim[1].uniform) && (D < 3 || sp.dim[2].uniform) &&
                              sp.dim[0].poly != nullptr;
-        if (uniform) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const int n = sp.dim[d].nclass * kCoef;
-                for (int i = threadIdx.x; i < n; i += blockDim.x) s_poly[d][i] = sp.dim[d].poly[i];
-            }
-        }
         __syncthreads();
         if (static_cast<int>(threadIdx.x) < w.count) {
             const int64_t pt = w.first + threadIdx.x;
             const bool lastc = !sp.dim[D - 1].periodic;
             if (uniform) {
                 if (lastc)
-                    uniform_point<D, K, false, true, double>(sp, s_poly, pt);
+                    uniform_point<D, K, false, true, double>(sp, pt);
                 else
-                    uniform_point<D, K, false, false, double>(sp, s_poly, pt);
+                    uniform_point<D, K, false, false, double>(sp, pt);
             } else {
                 if (lastc)
                     general_point<D, K, false, true, double>(sp, pt);

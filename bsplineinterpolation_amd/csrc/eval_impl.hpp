@@ -251,26 +251,26 @@ __device__ __forceinline__ void poly_weights(PP P, double u, double inv_dx, int 
 }
 
 // Returns the span; fills the weights and the HBM offsets of the stencil.
-// `poly` is the LDS copy of the dimension's polynomial classes; when every
-// lane of the wave is on an interior span (class 0, the common case) the
-// coefficients are read with scalar loads from dp.poly instead (the same
-// values), which keeps them out of the LDS pipe.
+// The basis polynomials of the span's class come from dp.poly: with scalar
+// loads when every lane of the wave is on an interior span (class 0, the
+// common case), otherwise with vector loads of each lane's class (a few KB,
+// cached).  No LDS copy: staging the classes in every workgroup cost more
+// than it saved (up to 9.5 KB per workgroup at order 5).
 template <int K, bool GRAD, typename W>
-__device__ __forceinline__ int uniform_weights(const DimParams& dp, const double* __restrict__ poly, double x,
-                                               int r, int* bad, DimW<K, uint32_t, W>& dd) {
+__device__ __forceinline__ int uniform_weights(const DimParams& dp, double x, int r, int* bad,
+                                               DimW<K, uint32_t, W>& dd) {
     wrap_or_check(dp, x, bad);
     const int last = dp.nt - K - 2;
     const int mu = uniform_span<K>(dp, x);
     const int cls = (mu - K < dp.nleft) ? 1 + (mu - K) : ((last - mu < dp.nright) ? 1 + dp.nleft + (last - mu) : 0);
     const double u = (x - uknot_sel<K>(dp, mu)) * dp.inv_dx;
-    // explicit address spaces: merged into one generic pointer, both
-    // branches would read through flat loads (LDS at memory latency, and
-    // no scalar loads for the uniform case)
+    // explicit address spaces: as one generic pointer both branches would
+    // read through flat loads (and the uniform case without scalar loads)
     if (__all(cls == 0))
         poly_weights<K, GRAD, W>((const __attribute__((address_space(4))) double*)dp.poly, u, dp.inv_dx, r, dd);
     else
-        poly_weights<K, GRAD, W>((const __attribute__((address_space(3))) double*)(poly + cls * (K + 1) * (K + 1)), u,
-                                 dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W>((const __attribute__((address_space(1))) double*)(dp.poly + cls * (K + 1) * (K + 1)),
+                                 u, dp.inv_dx, r, dd);
     stencil_offsets<K>(dp, mu, dd.off);
     return mu;
 }
@@ -476,8 +476,7 @@ __global__ __launch_bounds__(kEvalBlock) void eval_general_kernel(EvalParams p) 
 }
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
-__device__ __forceinline__ void uniform_point(const EvalParams& p, const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
-                                              int64_t pt) {
+__device__ __forceinline__ void uniform_point(const EvalParams& p, int64_t pt) {
     const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
     DimW<K, uint32_t, T> dd[D];
@@ -486,7 +485,7 @@ __device__ __forceinline__ void uniform_point(const EvalParams& p, const double 
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+        uniform_weights<K, GRAD, T>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
     }
     if constexpr (GRAD) {
         T res[D + 1];
@@ -500,17 +499,8 @@ __device__ __forceinline__ void uniform_point(const EvalParams& p, const double 
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
 __global__ __launch_bounds__(kEvalBlock) void eval_uniform_kernel(EvalParams p) {
-    constexpr int kClassMax = 1 + 2 * K;
-    constexpr int kCoef = (K + 1) * (K + 1);
-    __shared__ double s_poly[D][kClassMax * kCoef];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int n = p.dim[d].nclass * kCoef;
-        for (int i = threadIdx.x; i < n; i += kEvalBlock) s_poly[d][i] = p.dim[d].poly[i];
-    }
-    __syncthreads();
     const int64_t pt = block_index() * kEvalBlock + threadIdx.x;
-    if (pt < p.npts) uniform_point<D, K, GRAD, LASTC, T>(p, s_poly, pt);
+    if (pt < p.npts) uniform_point<D, K, GRAD, LASTC, T>(p, pt);
 }
 
 // ---- grouped evaluation with the brick patch in LDS ---------------------
@@ -538,9 +528,7 @@ constexpr int patch_max() {
 // (its brick is the chunk's), the stencil is read from LDS, otherwise from
 // HBM.  Same weights and contraction order either way.
 template <int D, int K, bool GRAD, bool LASTC, typename T>
-__device__ __forceinline__ void patch_point(const EvalParams& p,
-                                            const double (*s_poly)[(1 + 2 * K) * (K + 1) * (K + 1)],
-                                            const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
+__device__ __forceinline__ void patch_point(const EvalParams& p, const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
                                             const uint32_t (&lstride)[D], const BinRecord& rec) {
     const int64_t opt = static_cast<int64_t>(rec.idx);
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
@@ -551,7 +539,7 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], s_poly[d], rec.x[d], r,
+        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], rec.x[d], r,
                                                    p.bad, dd[d]);
         const int n = static_cast<int>(p.dim[d].n);
         int st = mu - K;  // first stencil point; periodic even orders: may be n (wraps to 0)
@@ -589,17 +577,9 @@ __device__ __forceinline__ void patch_point(const EvalParams& p,
 // past the end of clamped ones, where no stencil reaches), then evaluates.
 template <int D, int K, bool GRAD, bool LASTC, typename T>
 __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
-    constexpr int kClassMax = 1 + 2 * K;
-    constexpr int kCoef = (K + 1) * (K + 1);
     constexpr int kPatchMax = patch_max<T>();
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
-    __shared__ double s_poly[D][kClassMax * kCoef];
     __shared__ T s_patch[kPatchMax];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const int n = p.dim[d].nclass * kCoef;
-        for (int i = threadIdx.x; i < n; i += kEvalBlock) s_poly[d][i] = p.dim[d].poly[i];
-    }
     const int64_t first = block_index() * kEvalBlock;
     const int64_t pivot = min(first + kEvalBlock / 2, p.npts - 1);
     const int64_t pt = first + threadIdx.x;
@@ -692,7 +672,7 @@ __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
     // the thread's grouped record (coordinates, caller position); loaded
     // after the barrier (issued before the staging, it measured slower)
     if (pt < p.npts)
-        patch_point<D, K, GRAD, LASTC, T>(p, s_poly, s_patch, bstart, bsize, lstride,
+        patch_point<D, K, GRAD, LASTC, T>(p, s_patch, bstart, bsize, lstride,
                                           load_record(reinterpret_cast<const BinRecord*>(p.coords) + pt));
 }
 
