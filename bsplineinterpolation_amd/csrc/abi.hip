@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <list>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -221,6 +222,95 @@ int make_grid(intp_ctx ctx, const intp_grid_desc* desc, hipStream_t stream, bool
     if (own_stream) HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     out = std::move(g);
     return INTP_OK;
+}
+
+// ---- grids of recent builds ---------------------------------------------
+//
+// intp_build / intp_build_refined on a uniform grid equal to a recent
+// build's (same device, dimension, order, sizes, periodicity and ranges)
+// reuse that grid's uploaded tables instead of recomputing and uploading
+// them: the tables are a deterministic function of the descriptor, so the
+// results do not change.  A few small grids are kept (grids whose tables
+// exceed kGridCacheBytes are never cached); a spline keeps its grid alive
+// whether or not the cache still holds it.
+constexpr size_t kGridCacheEntries = 16;
+constexpr size_t kGridCacheBytes = size_t{8} << 20;
+
+struct GridKey {
+    int device = 0, dim = 0, order = 0;
+    int64_t n[kMaxDim] = {};
+    int32_t periodic[kMaxDim] = {};
+    double xmin[kMaxDim] = {}, xmax[kMaxDim] = {};
+    bool operator==(const GridKey& o) const {
+        if (device != o.device || dim != o.dim || order != o.order) return false;
+        for (int d = 0; d < dim; ++d)
+            if (n[d] != o.n[d] || periodic[d] != o.periodic[d] || std::memcmp(&xmin[d], &o.xmin[d], 8) != 0 ||
+                std::memcmp(&xmax[d], &o.xmax[d], 8) != 0)
+                return false;
+        return true;
+    }
+};
+
+// Never destroyed: its grids would otherwise be freed by static
+// destructors, after the HIP runtime may have shut down.
+std::mutex g_grid_cache_mu;
+std::list<std::pair<GridKey, std::shared_ptr<GridData>>>& g_grid_cache =
+    *new std::list<std::pair<GridKey, std::shared_ptr<GridData>>>();  // front: most recent
+
+// The key of a uniform descriptor; false for non-uniform grids (not cached).
+bool grid_key(int device, const intp_grid_desc& desc, GridKey* key) {
+    key->device = device;
+    key->dim = desc.dim;
+    key->order = desc.order;
+    for (int d = 0; d < desc.dim; ++d) {
+        if (desc.coords[d]) return false;
+        key->n[d] = desc.n[d];
+        key->periodic[d] = desc.periodic[d] != 0;
+        key->xmin[d] = desc.xmin[d];
+        key->xmax[d] = desc.xmax[d];
+    }
+    return true;
+}
+
+// A cached grid for `key`, made usable on `stream` (which waits for its
+// upload), or null.
+std::shared_ptr<GridData> cached_grid(const GridKey& key, hipStream_t stream) {
+    std::shared_ptr<GridData> g;
+    {
+        std::lock_guard<std::mutex> lk(g_grid_cache_mu);
+        for (auto it = g_grid_cache.begin(); it != g_grid_cache.end(); ++it)
+            if (it->first == key) {
+                g = it->second;
+                g_grid_cache.splice(g_grid_cache.begin(), g_grid_cache, it);
+                break;
+            }
+    }
+    if (g && g->uploaded && hipStreamWaitEvent(stream, g->uploaded, 0) != hipSuccess) return nullptr;
+    return g;
+}
+
+void remember_grid(const GridKey& key, const std::shared_ptr<GridData>& g) {
+    if (g->image.size() > kGridCacheBytes) return;
+    std::shared_ptr<GridData> evicted;  // released outside the lock
+    std::lock_guard<std::mutex> lk(g_grid_cache_mu);
+    g_grid_cache.emplace_front(key, g);
+    if (g_grid_cache.size() > kGridCacheEntries) {
+        evicted = std::move(g_grid_cache.back().second);
+        g_grid_cache.pop_back();
+    }
+}
+
+// make_grid for a build: a cached grid when the descriptor has one.
+int build_grid(intp_ctx ctx, const intp_grid_desc* desc, hipStream_t stream, std::shared_ptr<GridData>& out) {
+    GridKey key;
+    const bool cacheable = ctx && desc && desc->dim >= 1 && desc->dim <= kMaxDim && grid_key(ctx->device, *desc, &key);
+    if (cacheable) {
+        DeviceScope scope(ctx->device);
+        if (scope.ok() && (out = cached_grid(key, stream))) return INTP_OK;
+    }
+    const int rc = make_grid(ctx, desc, stream, false, out);
+    if (rc == INTP_OK && cacheable) remember_grid(key, out);
+    return rc;
 }
 
 // Periodic index shift of the right-hand sides along each dimension
@@ -553,7 +643,7 @@ int intp_build_refined(intp_ctx ctx, const intp_grid_desc* desc, const double* v
                        int32_t sweeps, intp_spline* out, void* stream) {
     if (!out) return fail(INTP_E_INVALID, "null output handle");
     intp_template_s t;
-    const int rc = make_grid(ctx, desc, static_cast<hipStream_t>(stream), false, t.grid);
+    const int rc = build_grid(ctx, desc, static_cast<hipStream_t>(stream), t.grid);
     if (rc != INTP_OK) return rc;
     return intp_interpolate_refined(&t, values, values_mem, sweeps, out, stream);
 }
@@ -562,7 +652,7 @@ int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, i
                void* stream) {
     if (!out) return fail(INTP_E_INVALID, "null output handle");
     intp_template_s t;
-    const int rc = make_grid(ctx, desc, static_cast<hipStream_t>(stream), false, t.grid);
+    const int rc = build_grid(ctx, desc, static_cast<hipStream_t>(stream), t.grid);
     if (rc != INTP_OK) return rc;
     return intp_interpolate(&t, values, values_mem, out, stream);
 }

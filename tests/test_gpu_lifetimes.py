@@ -47,6 +47,39 @@ def test_gpu_release_while_queued(amd, orc):
             assert rel_err(got[i, idx], want[i]) < 1e-12
 
 
+def test_gpu_builds_sharing_a_grid_on_two_streams(amd, orc):
+    """Builds on one uniform grid reuse its uploaded tables; a build issued
+    on a second stream right behind the first (which uploaded them) waits
+    for that upload, and each spline stays correct after the other and the
+    grid's first user are released."""
+    import torch
+    rng = np.random.default_rng(13)
+    shape = (70, 90, 33)
+    rg = [(0.0, 1.0), (-2.0, 2.0), (0.5, 1.5)]
+    per = [False, True, False]
+    fs = [rng.standard_normal(shape) for _ in range(3)]
+    a, b = torch.cuda.Stream(), torch.cuda.Stream()
+    # a fresh range, so the first build uploads the grid
+    rg[0] = (0.0, 1.0 + rng.uniform(0.1, 0.2))
+    fds = [torch.from_numpy(f).cuda() for f in fs]
+    torch.cuda.synchronize()
+    with torch.cuda.stream(a):
+        g0 = amd.InterpolationFunction(5, fds[0], periodic=per, ranges=rg, stream=a)
+    with torch.cuda.stream(b):
+        g1 = amd.InterpolationFunction(5, fds[1], periodic=per, ranges=rg, stream=b)
+        g2 = amd.InterpolationFunction(5, fds[2], periodic=per, ranges=rg, stream=b, precision="f32_refined")
+    q = np.stack([rng.uniform(r[0], r[1], 3000) for r in rg])
+    qd = torch.from_numpy(q).cuda()
+    b.synchronize()
+    del g0
+    outs = [g1.eval(qd), g2.eval(qd)]
+    torch.cuda.synchronize()
+    # refined FP32 on noise data: the refined tests' bar for rough values
+    for f, o, bar in ((fs[1], outs[0], 1e-12), (fs[2], outs[1], 1e-5)):
+        want = orc.OracleSpline(5, f, periodic=per, ranges=rg).eval(np.ascontiguousarray(q.T), nthreads=8)
+        assert rel_err(o.cpu().numpy()[0], want[0]) < bar
+
+
 def test_gpu_build_and_eval_on_two_streams(amd, orc):
     import torch
     rng = np.random.default_rng(12)
