@@ -10,7 +10,7 @@
 //    with the host's exact operation order (no FMA), so the span test is
 //    the reference's, with no memory traffic; the basis comes from the
 //    span's polynomial coefficients (one class for all interior spans,
-//    one per clamped end span), staged in LDS;
+//    one per clamped end span), read from the device table;
 //  * other grids: knots and reciprocal knot gaps are read from HBM and the
 //    Cox-de Boor triangle (BSpline.hpp:83-111) runs in registers.
 // Derivatives: the reference differentiates the gathered control points
