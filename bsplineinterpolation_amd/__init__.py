@@ -193,19 +193,25 @@ class InterpolationFunction:
 
     # ---- evaluation ---------------------------------------------------
     def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None, timed: bool = False,
-             out_dtype=None):
+             out_dtype=None, group: Optional[bool] = None):
         """Batched evaluation.  ``points``: numpy [npts, D] (host) or a
         float64 torch tensor [D, npts] (SoA) / [npts, D] (AoS) on the
         device.  ``orders``: [nout, D] derivative orders (default: value).
         Returns [nout, npts] (numpy or torch, like ``points``), float64, or
         float32 when ``out`` is float32 or ``out_dtype`` says so.  ``timed``
-        records device events around the call (read with eval_timings())."""
+        records device events around the call (read with eval_timings()).
+        ``group``: True / False forces / forbids grouping the points by
+        brick (INTP_EVAL_GROUPED / INTP_EVAL_UNGROUPED); None lets the
+        library decide from the batch and table sizes.  Results are the
+        same either way."""
         lib = _lib.load()
         D = self.dim
         orders = np.zeros((1, D), np.int32) if orders is None else \
             np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
         nout = orders.shape[0]
         flags = (_lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0) | (_lib.INTP_EVAL_TIMED if timed else 0)
+        if group is not None:
+            flags |= _lib.INTP_EVAL_GROUPED if group else _lib.INTP_EVAL_UNGROUPED
         op = orders.ctypes.data_as(C.POINTER(C.c_int32))
         if _is_torch(points):
             import torch

@@ -29,6 +29,7 @@ INTP_EVAL_HOST_MEM = 2
 INTP_EVAL_UNGROUPED = 4
 INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
+INTP_EVAL_GROUPED = 32
 INTP_BATCH_GRAPH = 1
 INTP_BATCH_AUTOTUNE = 2
 INTP_BATCH_SPLIT_LINES = 4

@@ -471,8 +471,21 @@ DimParams dim_params(const GridData& g, int d) {
     return make_dim_params(g.setup[d], g.order, stride, g.knots[d], g.recip[d], g.poly[d]);
 }
 
-// Batches at least this large are grouped by brick before evaluation.
+// Batches at least this large are grouped by brick before evaluation when
+// the table is large enough for grouping to pay (group_pays).
 constexpr int64_t kGroupMinPoints = int64_t{1} << 20;
+
+// Whether grouping pays for a batch (measured on MI355X, value + first
+// partials, cubic): 3D tables of 17 MB and more gain from 2^20 points on
+// (2^24 points: 1.4x at 128^3, 1.9x at 256^3); 2D tables up to 34 MB do
+// not (their (k+1)-row stencils stay cache-friendly: 2.5x faster
+// ungrouped at 1024^2 and 2^24 points), nor do 3D tables below 16 MB.
+// 2D tables beyond the 256 MB Infinity Cache are grouped.
+bool group_pays(int dim, int64_t table_bytes, int64_t npts) {
+    if (npts < kGroupMinPoints) return false;
+    if (dim == 3) return table_bytes >= (int64_t{16} << 20);
+    return table_bytes >= (int64_t{256} << 20);
+}
 
 // Bricks of B^D stencil starts, B a power of two: the largest B whose
 // (B + order)^D patch of table elements (elem_bytes each) stays within
@@ -764,6 +777,8 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     for (int i = 0; i < nout * D; ++i)
         if (orders[i] < 0) return fail(INTP_E_INVALID, "negative derivative order");
     if (cs_dim < 0 || cs_pt < 0 || os_out < 0 || os_pt < 0) return fail(INTP_E_INVALID, "negative stride");
+    if ((flags & INTP_EVAL_GROUPED) && (flags & INTP_EVAL_UNGROUPED))
+        return fail(INTP_E_INVALID, "INTP_EVAL_GROUPED and INTP_EVAL_UNGROUPED together");
     hipStream_t stream = static_cast<hipStream_t>(stream_);
     DeviceScope scope(g.device);
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
@@ -874,7 +889,9 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         if (rec_b) (void)hipFreeAsync(rec_b, stream);
         if (bin_scratch) (void)hipFreeAsync(bin_scratch, stream);
     };
-    if (fast && D >= 2 && npts >= kGroupMinPoints && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_UNGROUPED) &&
+    const bool group = (flags & INTP_EVAL_GROUPED) ? npts > 0
+                       : !(flags & INTP_EVAL_UNGROUPED) && group_pays(D, g.total * s->ctrl_bytes, npts);
+    if (fast && D >= 2 && group && npts <= 0xFFFFFFFFll &&
         bin_scan_tiles(npts) <= 4096) {
         int64_t patch_volume = 0;
         BinParams bp = bin_params(g, s->ctrl_bytes, &patch_volume);

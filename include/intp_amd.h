@@ -54,6 +54,9 @@ typedef enum {
 #define INTP_EVAL_HOST_MEM 2u     /* coords and outputs are host memory   */
 #define INTP_EVAL_UNGROUPED 4u    /* evaluate in the caller's point order:
                                      no grouping of large batches by brick */
+#define INTP_EVAL_GROUPED 32u     /* group by brick whenever the grid
+                                     allows it (uniform 2D/3D grids),
+                                     whatever the batch and table sizes    */
 #define INTP_EVAL_TIMED 8u        /* record HIP events around this call's
                                      grouping and its evaluation kernels,
                                      read back with intp_eval_timings()   */

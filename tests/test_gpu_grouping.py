@@ -2,7 +2,9 @@
 against the caller-order path and the oracle.
 
 Each point runs the same arithmetic in both paths, so the results must be
-bit-identical; a subsample is also checked against the oracle (1e-12)."""
+bit-identical; a subsample is also checked against the oracle (1e-12).
+Grouping is forced (INTP_EVAL_GROUPED): these tables are small enough that
+the library would otherwise evaluate them in caller order."""
 import math
 
 import numpy as np
@@ -24,7 +26,7 @@ def amd():
 def _eval_both(g, q, orders):
     import torch
     qd = torch.from_numpy(q).cuda()
-    a = g.eval(qd, orders).cpu().numpy()
+    a = g.eval(qd, orders, group=True).cpu().numpy()
     # same call, caller order
     import ctypes as C
     from bsplineinterpolation_amd import _lib
@@ -69,10 +71,23 @@ def test_gpu_grouped_host_buffers_and_bounds(amd):
     f = rng.standard_normal((64, 48, 40))
     g = amd.InterpolationFunction(3, f, ranges=[(0, 1), (0, 1), (0, 1)])
     q = rng.uniform(0, 1, ((1 << 20) + 5, 3))
-    host = g.eval(q, [[0, 0, 0], [0, 1, 0]])
+    host = g.eval(q, [[0, 0, 0], [0, 1, 0]], group=True)
     import torch
-    dev = g.eval(torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), [[0, 0, 0], [0, 1, 0]]).cpu().numpy()
+    dev = g.eval(torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), [[0, 0, 0], [0, 1, 0]], group=True).cpu().numpy()
     assert np.array_equal(host, dev)
+    # the library's own choice (caller order for this 1 MB table) gives the same bits
+    assert np.array_equal(host, g.eval(q, [[0, 0, 0], [0, 1, 0]]))
     q[777, 1] = 1.5
     with pytest.raises(amd.DomainError):
-        g.eval(q, bounds=True)
+        g.eval(q, bounds=True, group=True)
+    from bsplineinterpolation_amd import _lib
+    import ctypes as C
+    qd = torch.from_numpy(np.ascontiguousarray(q.T)).cuda()
+    out = torch.empty((1, q.shape[0]), dtype=torch.float64, device="cuda")
+    both = _lib.INTP_EVAL_GROUPED | _lib.INTP_EVAL_UNGROUPED
+    dp = C.POINTER(C.c_double)
+    o = np.zeros(3, np.int32)
+    rc = _lib.load().intp_eval(g.h, q.shape[0], C.cast(qd.data_ptr(), dp), q.shape[0], 1, 1,
+                               o.ctypes.data_as(C.POINTER(C.c_int32)), C.cast(out.data_ptr(), dp), q.shape[0], 1,
+                               both, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == _lib.INTP_E_INVALID

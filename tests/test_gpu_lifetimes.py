@@ -114,10 +114,10 @@ def test_gpu_eval_timings(amd):
     small = torch.from_numpy(rng.uniform(0, 1, (3, 1000))).cuda()
     large = torch.from_numpy(rng.uniform(0, 1, (3, 1 << 20))).cuda()
     g.eval(small, timed=True)
-    g.eval(large, [[0, 0, 0], [1, 0, 0]], timed=True)
+    g.eval(large, [[0, 0, 0], [1, 0, 0]], timed=True, group=True)
     g.eval(small)  # not timed
     t = amd.eval_timings()
     assert len(t) == 2
     assert t[0][0] < 0.05 and t[0][1] > 0.0  # small batch: no grouping pass
-    assert t[1][0] > 0.0 and t[1][1] > 0.0   # large batch: grouped
+    assert t[1][0] > 0.0 and t[1][1] > 0.0   # large batch, grouping forced: grouped
     assert amd.eval_timings() == []

@@ -108,8 +108,8 @@ def test_gpu_refined_f32_outputs_and_grouping(amd):
     q = np.stack([rng.uniform(r[0], r[1], (1 << 20) + 9) for r in rg])
     qd = torch.from_numpy(q).cuda()
     orders = [[0, 0, 0], [0, 1, 0], [2, 0, 1]]
-    out64 = g.eval(qd, orders)                         # grouped, FP64 results of FP32 arithmetic
-    out32 = g.eval(qd, orders, out_dtype=torch.float32)
+    out64 = g.eval(qd, orders, group=True)             # grouped, FP64 results of FP32 arithmetic
+    out32 = g.eval(qd, orders, out_dtype=torch.float32, group=True)
     assert np.array_equal(out32.cpu().numpy(), out64.cpu().numpy().astype(np.float32))
     # caller-order evaluation gives the same bits
     import ctypes as C
