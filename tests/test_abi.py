@@ -30,6 +30,19 @@ def test_header_declares_expected_entry_points():
     assert _declared() == sorted(_lib.EXPORTS)
 
 
+def test_header_constants_match_the_python_binding():
+    """Every #define INTP_* number in the header has the same value in
+    the ctypes binding (flags, status codes, kinds)."""
+    from bsplineinterpolation_amd import _lib
+    defs = re.findall(r"^#define\s+(INTP_\w+)\s+(-?\d+)u?\b", open(HEADER).read(), re.M)
+    assert len(defs) > 20
+    for name, value in defs:
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == int(value), name
+    for name in ("INTP_EVAL_UNGROUPED", "INTP_EVAL_GROUPED", "INTP_EVAL_OUT_F32", "INTP_BATCH_SPLIT_LINES"):
+        assert hasattr(_lib, name), name
+
+
 def test_library_exports_every_declared_symbol(lib):
     from bsplineinterpolation_amd import _lib
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
