@@ -91,3 +91,19 @@ def test_gpu_grouped_host_buffers_and_bounds(amd):
                                o.ctypes.data_as(C.POINTER(C.c_int32)), C.cast(out.data_ptr(), dp), q.shape[0], 1,
                                both, C.c_void_p(torch.cuda.current_stream().cuda_stream))
     assert rc == _lib.INTP_E_INVALID
+
+
+@pytest.mark.parametrize("npts", [1, 7, 1000, 2049])
+def test_gpu_forced_grouping_of_small_batches(amd, npts):
+    """INTP_EVAL_GROUPED on batches far below the automatic threshold
+    (one partial chunk, a single point) gives the caller-order bits."""
+    import torch
+    rng = np.random.default_rng(npts)
+    f = rng.standard_normal((40, 36, 30))
+    g = amd.InterpolationFunction(3, f, periodic=[False, True, False], ranges=[(0, 1), (0, 2), (-1, 1)])
+    q = torch.from_numpy(np.stack([rng.uniform(-0.05, 1.05, npts), rng.uniform(0, 2, npts),
+                                   rng.uniform(-1, 1, npts)])).cuda()
+    orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    a = g.eval(q, orders, group=True).cpu().numpy()
+    b = g.eval(q, orders, group=False).cpu().numpy()
+    assert np.array_equal(a, b)
