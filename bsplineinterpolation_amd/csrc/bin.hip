@@ -132,12 +132,18 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
 // (local rank from an LDS atomic, local start from a block scan of the
 // chunk's digit counts), then written out from LDS so that consecutive
 // lanes write consecutive slots of one digit's run.
+// The chunk goes out in kStageParts rounds of kChunk / kStageParts sorted
+// positions, so the LDS stage is that much smaller (more workgroups per
+// CU) while each digit's run still spans the whole chunk.
+constexpr int kStageParts = 2;
+
 template <int PASS>
 __global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, const uint32_t* __restrict__ offs) {
+    constexpr uint32_t kPart = kChunk / kStageParts;
     __shared__ uint32_t cnt[kRadix];
     __shared__ uint32_t gbase[kRadix];
     __shared__ uint32_t lstart[kRadix];
-    __shared__ BinRecord stage[kChunk];
+    __shared__ BinRecord stage[kPart];
     const int64_t chunk = xcd_chunk_index();
     cnt[threadIdx.x] = 0;
     gbase[threadIdx.x] = offs[static_cast<int64_t>(threadIdx.x) * p.nblocks + chunk];
@@ -166,19 +172,30 @@ __global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, cons
     uint32_t total;
     lstart[threadIdx.x] = block_exclusive_scan(cnt[threadIdx.x], &total);
     __syncthreads();
+    const uint32_t n = static_cast<uint32_t>(min<int64_t>(kChunk, p.npts - first));
+    uint32_t pos_of[kPerThread];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        const int64_t i = first + j * kThreads + threadIdx.x;
-        if (i < p.npts) stage[lstart[digit[j]] + rank[j]] = r[j];
-    }
-    __syncthreads();
-    const int n = static_cast<int>(min<int64_t>(kChunk, p.npts - first));
-    for (int pos = threadIdx.x; pos < n; pos += kThreads) {
-        const BinRecord rec = stage[pos];
-        const uint32_t d = PASS == 0 ? (rec.key & 0xFFu) : ((rec.key >> 8) & 0xFFu);
-        const uint32_t slot = gbase[d] + (pos - lstart[d]);
-        p.rec_out[slot] = rec;
-        if constexpr (PASS == 0) p.key_out[slot] = rec.key;
+    for (int j = 0; j < kPerThread; ++j) pos_of[j] = lstart[digit[j]] + rank[j];
+#pragma unroll
+    for (int part = 0; part < kStageParts; ++part) {
+        const uint32_t lo = part * kPart;
+        if (lo >= n) break;  // uniform across the workgroup
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+            const int64_t i = first + j * kThreads + threadIdx.x;
+            if (i < p.npts && pos_of[j] >= lo && pos_of[j] < lo + kPart) stage[pos_of[j] - lo] = r[j];
+        }
+        __syncthreads();
+        const uint32_t m = min(n - lo, kPart);
+        for (uint32_t q = threadIdx.x; q < m; q += kThreads) {
+            const BinRecord rec = stage[q];
+            const uint32_t pos = lo + q;
+            const uint32_t d = PASS == 0 ? (rec.key & 0xFFu) : ((rec.key >> 8) & 0xFFu);
+            const uint32_t slot = gbase[d] + (pos - lstart[d]);
+            p.rec_out[slot] = rec;
+            if constexpr (PASS == 0) p.key_out[slot] = rec.key;
+        }
+        __syncthreads();
     }
 }
 
