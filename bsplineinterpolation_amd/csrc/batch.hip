@@ -44,12 +44,19 @@ namespace {
 // Rows per substitution batch in the merged kernels (bounds registers:
 // two workgroups per CU).
 constexpr int kMergedRows = 4;
+// Small tiles (at most kSmallTileBytes of LDS) are solved by one-wave
+// workgroups, so that every resident wave substitutes a tile of its own
+// (a 256-thread workgroup has one substituting wave and three waiting at
+// the barrier); larger tiles and the in-place runs keep 256 threads, whose
+// four waves share the loads and stores of a large tile.
+constexpr int kSmallSolveThreads = 64;
+constexpr int64_t kSmallTileBytes = 20 * 1024;
 constexpr int kLineThreads = 64;
 constexpr int kMaxBatchBand = 6;
 // 1D splines of at least this many points are split (INTP_BATCH_SPLIT_LINES)
 constexpr int64_t kSplitMinRows = 256;
 
-template <int BW, bool CYC>
+template <int BW, bool CYC, int NT>
 __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const BatchSolveItem& w, const double* values, double* ctrl,
                                                  int32_t first_pass, unsigned char* tile_bytes, int64_t* dst_base,
                                                  int64_t* src_base) {
@@ -80,7 +87,7 @@ __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const
             }
             sp.shift_last = b.shift[a];
         }
-        solve_tile<BW, CYC, double, double, kMergedRows>(sp, w.first_line, tile_bytes, dst_base, src_base);
+        solve_tile<BW, CYC, double, double, kMergedRows, NT>(sp, w.first_line, tile_bytes, dst_base, src_base);
         return;
     }
     // kSolveLanes: one line per lane, in place
@@ -375,7 +382,8 @@ __global__ __launch_bounds__(kLineThreads) void batch_line_split_kernel(const Ba
 // (tiles of lines of one spline along one axis, of any band width; the
 // items are ordered longest line first, so the long sequential lines
 // start at once and the short tiles fill in around them).
-__global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSpline* specs,
+template <int NT>
+__global__ __launch_bounds__(NT, NT == kSmallSolveThreads ? 2 : 1) void batch_solve_kernel(const BatchSpline* specs,
                                                                    const BatchSolveItem* items, int32_t nitems,
                                                                    const double* values, double* ctrl,
                                                                    int32_t first_pass) {
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(kTileThreads) void batch_solve_kernel(const BatchSp
         const BatchSolveItem w = items[it];
 #define INTP_SOLVE_CASE(BW, CYC)                                                                                  \
     case 2 * BW + CYC:                                                                                            \
-        batch_solve_item<BW, CYC>(specs, w, values, ctrl, first_pass, tile_bytes, dst_base, src_base);            \
+        batch_solve_item<BW, CYC, NT>(specs, w, values, ctrl, first_pass, tile_bytes, dst_base, src_base);        \
         break;
         // periodic band widths are floor(k/2) <= 3, clamped ones k - 1 <= 6
         switch (w.kind & 15) {
@@ -424,6 +432,7 @@ struct SolveGroup {
     int pass;
     int64_t offset, count;
     size_t lds;
+    int threads;  // kSmallSolveThreads or kTileThreads
 };
 
 struct EvalGroup {
@@ -434,7 +443,7 @@ struct EvalGroup {
 // Tunables (picked by the autotuner).
 struct Tuning {
     int tile_cap = 64;           // lines per solve tile, at most
-    int solve_blocks_per_cu = 4;  // persistent solve workgroups per CU
+    int solve_blocks_per_cu = 4;  // persistent 256-thread solve workgroups per CU
     int eval_threads = 256;       // points per eval item (= workgroup size)
     int eval_blocks_per_cu = 8;   // persistent eval workgroups per CU
 };
@@ -469,6 +478,8 @@ struct intp_batch_s {
     bool split_ready = false;
     hipStream_t side = nullptr;        // runs the 1D splines beside the passes
     hipStream_t side2 = nullptr;       // the split 1D lines, beside both
+    hipStream_t solve_side = nullptr;  // a pass's first solve group, beside its second
+    hipEvent_t solve_fork = nullptr, solve_join = nullptr;
     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
     // the (dimension, order) eval classes run side by side on these
     hipStream_t eval_side[kEvalStreams] = {};
@@ -511,6 +522,12 @@ struct intp_batch_s {
             (void)hipStreamSynchronize(side2);
             (void)hipStreamDestroy(side2);
         }
+        if (solve_side) {
+            (void)hipStreamSynchronize(solve_side);
+            (void)hipStreamDestroy(solve_side);
+        }
+        if (solve_fork) (void)hipEventDestroy(solve_fork);
+        if (solve_join) (void)hipEventDestroy(solve_join);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
         if (join2) (void)hipEventDestroy(join2);
@@ -552,10 +569,14 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
     auto kind_of = [](const DimSetup& ds, int mode) {
         return static_cast<int16_t>((mode << 4) | (2 * ds.lu.p + (ds.lu.cyclic ? 1 : 0)));
     };
-    std::vector<std::pair<int64_t, BatchSolveItem>> pass_items;
+    // per pass: the large tiles and in-place runs (256-thread workgroups),
+    // and the small tiles (one-wave workgroups), each longest first
+    std::vector<std::pair<int64_t, BatchSolveItem>> pass_items, small_items;
     for (int pass = 0; pass < kMaxDim; ++pass) {
-        SolveGroup g{pass, static_cast<int64_t>(b->solve_host.size()), 0, 0};
+        SolveGroup g{pass, 0, 0, 0, kTileThreads};
+        SolveGroup gs{pass, 0, 0, 0, kSmallSolveThreads};
         pass_items.clear();
+        small_items.clear();
         for (int s = 0; s < b->count; ++s) {
             const int D = b->dims[static_cast<size_t>(s)];
             if (D <= pass) continue;
@@ -575,16 +596,24 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
                     pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveLanes), first}});
             } else {
                 const int64_t lpb = bs.lines_per_block[a];
+                const int64_t bytes = lpb * (ds.n | 1) * 8;
+                const bool small = bytes <= kSmallTileBytes;
                 for (int64_t first = 0; first < nlines; first += lpb)
-                    pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveTile), first}});
-                g.lds = std::max(g.lds, static_cast<size_t>(lpb * (ds.n | 1) * 8));
+                    (small ? small_items : pass_items)
+                        .push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveTile), first}});
+                SolveGroup& to = small ? gs : g;
+                to.lds = std::max(to.lds, static_cast<size_t>(bytes));
             }
         }
-        std::stable_sort(pass_items.begin(), pass_items.end(),
-                         [](const auto& x, const auto& y) { return x.first > y.first; });
-        for (const auto& pi : pass_items) b->solve_host.push_back(pi.second);
-        g.count = static_cast<int64_t>(b->solve_host.size()) - g.offset;
-        if (g.count > 0) b->solve_groups.push_back(g);
+        for (auto* items : {&pass_items, &small_items}) {
+            SolveGroup& grp = items == &pass_items ? g : gs;
+            std::stable_sort(items->begin(), items->end(),
+                             [](const auto& x, const auto& y) { return x.first > y.first; });
+            grp.offset = static_cast<int64_t>(b->solve_host.size());
+            for (const auto& pi : *items) b->solve_host.push_back(pi.second);
+            grp.count = static_cast<int64_t>(b->solve_host.size()) - grp.offset;
+            if (grp.count > 0) b->solve_groups.push_back(grp);
+        }
     }
     const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->solve_host.size()));
     if (need > b->solve_cap) {
@@ -736,12 +765,55 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
         HIP_TRY(hipGetLastError(), "batch line launch");
         HIP_TRY(hipEventRecord(b->join, b->side), "hipEventRecord");
     }
-    for (const SolveGroup& g : b->solve_groups) {
-        const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
-        hipLaunchKernelGGL(batch_solve_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kTileThreads), g.lds, stream,
-                           b->specs, b->solve_items + g.offset, static_cast<int32_t>(g.count), values, b->ctrl,
-                           g.pass == 0 ? 1 : 0);
+    if (!b->solve_side) {
+        HIP_TRY(hipStreamCreateWithFlags(&b->solve_side, hipStreamNonBlocking), "hipStreamCreate");
+        HIP_TRY(hipEventCreateWithFlags(&b->solve_fork, hipEventDisableTiming), "hipEventCreate");
+        HIP_TRY(hipEventCreateWithFlags(&b->solve_join, hipEventDisableTiming), "hipEventCreate");
+    }
+    // a pass's two groups run side by side (they touch different splines);
+    // the next pass starts after both
+    for (size_t i = 0; i < b->solve_groups.size(); ++i) {
+        const SolveGroup& g = b->solve_groups[i];
+        const bool pair = i + 1 < b->solve_groups.size() && b->solve_groups[i + 1].pass == g.pass;
+        hipStream_t s = stream;
+        if (pair) {
+            HIP_TRY(hipEventRecord(b->solve_fork, stream), "hipEventRecord");
+            HIP_TRY(hipStreamWaitEvent(b->solve_side, b->solve_fork, 0), "hipStreamWaitEvent");
+            s = b->solve_side;
+        }
+        const int32_t first_pass = g.pass == 0 ? 1 : 0;
+        if (g.threads == kSmallSolveThreads) {
+            // one wave each: 8 resident per CU at 2 waves per SIMD
+            const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * 16));
+            hipLaunchKernelGGL(batch_solve_kernel<kSmallSolveThreads>, dim3(static_cast<unsigned>(blocks)),
+                               dim3(kSmallSolveThreads), g.lds, s, b->specs, b->solve_items + g.offset,
+                               static_cast<int32_t>(g.count), values, b->ctrl, first_pass);
+        } else {
+            const int blocks =
+                static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
+            hipLaunchKernelGGL(batch_solve_kernel<kTileThreads>, dim3(static_cast<unsigned>(blocks)),
+                               dim3(kTileThreads), g.lds, s, b->specs, b->solve_items + g.offset,
+                               static_cast<int32_t>(g.count), values, b->ctrl, first_pass);
+        }
         HIP_TRY(hipGetLastError(), "batch solve launch");
+        if (pair) {
+            HIP_TRY(hipEventRecord(b->solve_join, b->solve_side), "hipEventRecord");
+            const SolveGroup& h = b->solve_groups[++i];  // the pass's other group, on `stream`
+            if (h.threads == kSmallSolveThreads) {
+                const int blocks = static_cast<int>(std::min<int64_t>(h.count, int64_t{b->num_cus} * 16));
+                hipLaunchKernelGGL(batch_solve_kernel<kSmallSolveThreads>, dim3(static_cast<unsigned>(blocks)),
+                                   dim3(kSmallSolveThreads), h.lds, stream, b->specs, b->solve_items + h.offset,
+                                   static_cast<int32_t>(h.count), values, b->ctrl, first_pass);
+            } else {
+                const int blocks =
+                    static_cast<int>(std::min<int64_t>(h.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
+                hipLaunchKernelGGL(batch_solve_kernel<kTileThreads>, dim3(static_cast<unsigned>(blocks)),
+                                   dim3(kTileThreads), h.lds, stream, b->specs, b->solve_items + h.offset,
+                                   static_cast<int32_t>(h.count), values, b->ctrl, first_pass);
+            }
+            HIP_TRY(hipGetLastError(), "batch solve launch");
+            HIP_TRY(hipStreamWaitEvent(stream, b->solve_join, 0), "hipStreamWaitEvent");
+        }
     }
     if (lines) HIP_TRY(hipStreamWaitEvent(stream, b->join, 0), "hipStreamWaitEvent");
     if (lines && split && !b->split_host.empty()) HIP_TRY(hipStreamWaitEvent(stream, b->join2, 0), "hipStreamWaitEvent");

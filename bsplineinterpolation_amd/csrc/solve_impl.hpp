@@ -283,7 +283,7 @@ __host__ __device__ __forceinline__ int64_t tile_stride(int64_t n) { return n | 
 // One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS in
 // the compute type C; the table holds T.  With sp.accum set, the solved
 // lines are added into accum instead of being stored to x.
-template <int BW, bool CYC, typename T, typename C, int B = batch_rows<BW, CYC>()>
+template <int BW, bool CYC, typename T, typename C, int B = batch_rows<BW, CYC>(), int NT = kTileThreads>
 __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first, unsigned char* tile_bytes,
                                            int64_t* dst_base, int64_t* src_base) {
     C* tile = reinterpret_cast<C*>(tile_bytes);
@@ -319,19 +319,19 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
 
     // ---- load ----
     // Contiguous lines: tile element e is (line e / n, element e % n).
-    // Strided lines: (line e % lpb, element e / lpb); as 256 % lpb == 0 a
+    // Strided lines: (line e % lpb, element e / lpb); as NT % lpb == 0 a
     // thread keeps one line.
     const uint32_t un = static_cast<uint32_t>(n);
     const int lshift = __builtin_ctz(static_cast<unsigned>(lpb));
     const uint32_t count = static_cast<uint32_t>(inner == 1 ? nl * n : lpb * n);
     const uint32_t my_l = static_cast<uint32_t>(tid) & static_cast<uint32_t>(lpb - 1);
     if (sp.src) {
-        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+        for (uint32_t e0 = tid; e0 < count; e0 += NT * kBatch) {
             int64_t from[kBatch];
             uint32_t at[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t e = min(e0 + u * kTileThreads, count - 1);
+                const uint32_t e = min(e0 + u * NT, count - 1);
                 const uint32_t l = e / un, jj = e - l * un;
                 int64_t js = static_cast<int64_t>(jj) - sp.shift_last;
                 if (js < 0) js += n;
@@ -345,17 +345,17 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
             for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(__builtin_nontemporal_load(&sp.src[from[u]]));
 #pragma unroll
             for (int u = 0; u < kBatch; ++u)
-                if (e0 + u * kTileThreads < count) tile[at[u]] = v[u];
+                if (e0 + u * NT < count) tile[at[u]] = v[u];
         }
     } else if (inner == 1) {
         const T* __restrict__ base = X + first * n;
-        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+        for (uint32_t e0 = tid; e0 < count; e0 += NT * kBatch) {
             C v[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(base[min(e0 + u * kTileThreads, count - 1)]);
+            for (int u = 0; u < kBatch; ++u) v[u] = static_cast<C>(base[min(e0 + u * NT, count - 1)]);
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t e = e0 + u * kTileThreads;
+                const uint32_t e = e0 + u * NT;
                 if (e < count) {
                     const uint32_t l = e / un, jj = e - l * un;
                     tile[l * static_cast<uint32_t>(stride) + jj] = v[u];
@@ -365,16 +365,16 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
     } else if (static_cast<int>(my_l) < nl) {
         const T* __restrict__ line = X + dst_base[my_l];
         C* __restrict__ row = tile + my_l * stride;
-        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+        for (uint32_t e0 = tid; e0 < count; e0 += NT * kBatch) {
             C v[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t jj = min(e0 + u * kTileThreads, count - 1) >> lshift;
+                const uint32_t jj = min(e0 + u * NT, count - 1) >> lshift;
                 v[u] = static_cast<C>(line[static_cast<int64_t>(jj) * inner]);
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t e = e0 + u * kTileThreads;
+                const uint32_t e = e0 + u * NT;
                 if (e < count) row[e >> lshift] = v[u];
             }
         }
@@ -389,31 +389,31 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
     const bool add = sp.accum != nullptr;
     if (inner == 1) {
         T* __restrict__ base = Y + first * n;
-        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+        for (uint32_t e0 = tid; e0 < count; e0 += NT * kBatch) {
             C v[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t e = min(e0 + u * kTileThreads, count - 1);
+                const uint32_t e = min(e0 + u * NT, count - 1);
                 const uint32_t l = e / un, jj = e - l * un;
                 v[u] = tile[l * static_cast<uint32_t>(stride) + jj];
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u)
-                if (e0 + u * kTileThreads < count) {
-                    T* at = base + e0 + u * kTileThreads;
+                if (e0 + u * NT < count) {
+                    T* at = base + e0 + u * NT;
                     *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
                 }
         }
     } else if (static_cast<int>(my_l) < nl) {
         T* __restrict__ line = Y + dst_base[my_l];
         const C* __restrict__ row = tile + my_l * stride;
-        for (uint32_t e0 = tid; e0 < count; e0 += kTileThreads * kBatch) {
+        for (uint32_t e0 = tid; e0 < count; e0 += NT * kBatch) {
             C v[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) v[u] = row[min(e0 + u * kTileThreads, count - 1) >> lshift];
+            for (int u = 0; u < kBatch; ++u) v[u] = row[min(e0 + u * NT, count - 1) >> lshift];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
-                const uint32_t e = e0 + u * kTileThreads;
+                const uint32_t e = e0 + u * NT;
                 if (e < count) {
                     T* at = line + static_cast<int64_t>(e >> lshift) * inner;
                     *at = add ? static_cast<T>(*at + static_cast<T>(v[u])) : static_cast<T>(v[u]);
