@@ -770,17 +770,7 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
         HIP_TRY(hipEventCreateWithFlags(&b->solve_fork, hipEventDisableTiming), "hipEventCreate");
         HIP_TRY(hipEventCreateWithFlags(&b->solve_join, hipEventDisableTiming), "hipEventCreate");
     }
-    // a pass's two groups run side by side (they touch different splines);
-    // the next pass starts after both
-    for (size_t i = 0; i < b->solve_groups.size(); ++i) {
-        const SolveGroup& g = b->solve_groups[i];
-        const bool pair = i + 1 < b->solve_groups.size() && b->solve_groups[i + 1].pass == g.pass;
-        hipStream_t s = stream;
-        if (pair) {
-            HIP_TRY(hipEventRecord(b->solve_fork, stream), "hipEventRecord");
-            HIP_TRY(hipStreamWaitEvent(b->solve_side, b->solve_fork, 0), "hipStreamWaitEvent");
-            s = b->solve_side;
-        }
+    auto launch_group = [&](const SolveGroup& g, hipStream_t s) -> hipError_t {
         const int32_t first_pass = g.pass == 0 ? 1 : 0;
         if (g.threads == kSmallSolveThreads) {
             // one wave each: 8 resident per CU at 2 waves per SIMD
@@ -795,25 +785,23 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
                                dim3(kTileThreads), g.lds, s, b->specs, b->solve_items + g.offset,
                                static_cast<int32_t>(g.count), values, b->ctrl, first_pass);
         }
-        HIP_TRY(hipGetLastError(), "batch solve launch");
-        if (pair) {
-            HIP_TRY(hipEventRecord(b->solve_join, b->solve_side), "hipEventRecord");
-            const SolveGroup& h = b->solve_groups[++i];  // the pass's other group, on `stream`
-            if (h.threads == kSmallSolveThreads) {
-                const int blocks = static_cast<int>(std::min<int64_t>(h.count, int64_t{b->num_cus} * 16));
-                hipLaunchKernelGGL(batch_solve_kernel<kSmallSolveThreads>, dim3(static_cast<unsigned>(blocks)),
-                                   dim3(kSmallSolveThreads), h.lds, stream, b->specs, b->solve_items + h.offset,
-                                   static_cast<int32_t>(h.count), values, b->ctrl, first_pass);
-            } else {
-                const int blocks =
-                    static_cast<int>(std::min<int64_t>(h.count, int64_t{b->num_cus} * b->tune.solve_blocks_per_cu));
-                hipLaunchKernelGGL(batch_solve_kernel<kTileThreads>, dim3(static_cast<unsigned>(blocks)),
-                                   dim3(kTileThreads), h.lds, stream, b->specs, b->solve_items + h.offset,
-                                   static_cast<int32_t>(h.count), values, b->ctrl, first_pass);
-            }
-            HIP_TRY(hipGetLastError(), "batch solve launch");
-            HIP_TRY(hipStreamWaitEvent(stream, b->solve_join, 0), "hipStreamWaitEvent");
+        return hipGetLastError();
+    };
+    // a pass's two groups run side by side (they touch different splines);
+    // the next pass starts after both
+    for (size_t i = 0; i < b->solve_groups.size(); ++i) {
+        const SolveGroup& g = b->solve_groups[i];
+        const bool pair = i + 1 < b->solve_groups.size() && b->solve_groups[i + 1].pass == g.pass;
+        if (!pair) {
+            HIP_TRY(launch_group(g, stream), "batch solve launch");
+            continue;
         }
+        HIP_TRY(hipEventRecord(b->solve_fork, stream), "hipEventRecord");
+        HIP_TRY(hipStreamWaitEvent(b->solve_side, b->solve_fork, 0), "hipStreamWaitEvent");
+        HIP_TRY(launch_group(g, b->solve_side), "batch solve launch");
+        HIP_TRY(hipEventRecord(b->solve_join, b->solve_side), "hipEventRecord");
+        HIP_TRY(launch_group(b->solve_groups[++i], stream), "batch solve launch");
+        HIP_TRY(hipStreamWaitEvent(stream, b->solve_join, 0), "hipStreamWaitEvent");
     }
     if (lines) HIP_TRY(hipStreamWaitEvent(stream, b->join, 0), "hipStreamWaitEvent");
     if (lines && split && !b->split_host.empty()) HIP_TRY(hipStreamWaitEvent(stream, b->join2, 0), "hipStreamWaitEvent");
