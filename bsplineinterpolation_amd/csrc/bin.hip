@@ -108,21 +108,31 @@ template <int PASS>
 __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kRadix];
     h[threadIdx.x] = 0;
-    __syncthreads();
     const int64_t chunk = xcd_chunk_index();
     const int64_t first = chunk * kChunk;
+    // every load of the thread is issued before the first digit is formed
+    // (a load per LDS atomic leaves one load in flight per thread)
+    double x[kPerThread][kMaxDim];
+    uint32_t key[kPerThread];
+#pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
         const int64_t i = first + j * kThreads + threadIdx.x;
-        if (i >= p.npts) break;
-        uint32_t digit;
         if constexpr (PASS == 0) {
-            double x[kMaxDim];
-            load_point(p, i, x);
-            digit = brick_key(p, x) & 0xFFu;
+#pragma unroll
+            for (int d = 0; d < kMaxDim; ++d) x[j][d] = 0.0;
+            if (i < p.npts) load_point(p, i, x[j]);
         } else {
-            digit = (p.key_in[i] >> 8) & 0xFFu;
+            key[j] = i < p.npts ? p.key_in[i] : 0u;
         }
-        atomicAdd(&h[digit], 1u);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t i = first + j * kThreads + threadIdx.x;
+        if (i < p.npts) {
+            const uint32_t digit = PASS == 0 ? brick_key(p, x[j]) & 0xFFu : (key[j] >> 8) & 0xFFu;
+            atomicAdd(&h[digit], 1u);
+        }
     }
     __syncthreads();
     hist[static_cast<int64_t>(threadIdx.x) * p.nblocks + chunk] = h[threadIdx.x];
@@ -236,21 +246,43 @@ __global__ __launch_bounds__(kThreads) void scan_tiles_kernel(uint32_t* __restri
 __global__ __launch_bounds__(kThreads) void scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n,
                                                               const uint32_t* __restrict__ tile_offs,
                                                               uint32_t* __restrict__ out) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     constexpr int kPer = kScanTile / kThreads;
     const int64_t first = static_cast<int64_t>(blockIdx.x) * kScanTile + static_cast<int64_t>(threadIdx.x) * kPer;
+    // a thread's kPer entries are 64-byte aligned: whole runs in range move
+    // as 16-byte loads and stores
+    const bool whole = first + kPer <= n;
     uint32_t v[kPer];
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        v[j] = first + j < n ? in[first + j] : 0u;
-        s += v[j];
+    for (int j = 0; j < kPer; j += 4) {
+        if (whole) {
+            const u32x4 q = *reinterpret_cast<const u32x4*>(in + first + j);
+            v[j] = q.x, v[j + 1] = q.y, v[j + 2] = q.z, v[j + 3] = q.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[j + u] = first + j + u < n ? in[first + j + u] : 0u;
+        }
     }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) s += v[j];
     uint32_t total;
     uint32_t run = block_exclusive_scan(s, &total) + tile_offs[blockIdx.x];
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        if (first + j < n) out[first + j] = run;
-        run += v[j];
+    for (int j = 0; j < kPer; j += 4) {
+        u32x4 q;
+        q.x = run, run += v[j];
+        q.y = run, run += v[j + 1];
+        q.z = run, run += v[j + 2];
+        q.w = run, run += v[j + 3];
+        if (whole) {
+            *reinterpret_cast<u32x4*>(out + first + j) = q;
+        } else {
+            if (first + j < n) out[first + j] = q.x;
+            if (first + j + 1 < n) out[first + j + 1] = q.y;
+            if (first + j + 2 < n) out[first + j + 2] = q.z;
+            if (first + j + 3 < n) out[first + j + 3] = q.w;
+        }
     }
 }
 
