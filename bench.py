@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-EVAL_KERNEL = "eval_patch_kernel<3, 3, true, true>"
+EVAL_KERNEL = "eval_patch_kernel<3, 3, true, true, double>"
 TRAFFIC_PROFILE = os.path.join("profiles", "bench_3d256_k3_hbm_traffic.md")
 
 
