@@ -220,8 +220,14 @@ struct MatvecParams {
     int32_t dim;
     int64_t dn[kMaxDim];
     int64_t dshift[kMaxDim];
-    FastDiv div_inner, div_n, div_dn[kMaxDim];  // set by launch_band_matvec
-    int32_t xcd_remap;                            // set by launch_band_matvec
+    // set by launch_band_matvec
+    FastDiv div_inner, div_n, div_dn[kMaxDim];
+    int32_t xcd_remap;
+    // row-run kernel (axes before the last): runs of rows along the axis
+    // per column, and the thread count (columns * runs)
+    int32_t runs;
+    FastDiv div_runs;
+    int64_t work;
 };
 hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s);
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s);

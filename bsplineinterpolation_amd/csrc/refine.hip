@@ -78,15 +78,96 @@ __global__ __launch_bounds__(kBlock) void band_matvec_kernel(MatvecParams p) {
     static_cast<TO*>(p.y)[e] = static_cast<TO>(acc);
 }
 
+// Row-run variant for the axes before the last (inner > 1): a thread
+// takes R consecutive rows j0..j0+R-1 of one column (o, i) and slides a
+// window of R + 2W loaded values along them, so an input element is
+// loaded (R + 2W) / R times instead of 2W + 1.  Consecutive threads take
+// consecutive i (coalesced).  Each output sums its products in the order
+// of band_matvec_kernel, so the results are the same.
+// Requires n > 2W: a periodic index then needs at most one wrap.
+template <int W, int R, typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void band_matvec_rows_kernel(MatvecParams p) {
+    const uint64_t blk = p.xcd_remap ? static_cast<uint64_t>(xcd_chunk_index()) : blockIdx.x;
+    const uint64_t w64 = blk * kBlock + threadIdx.x;
+    if (w64 >= static_cast<uint64_t>(p.work)) return;
+    const uint32_t w = static_cast<uint32_t>(w64);
+    const uint32_t n = static_cast<uint32_t>(p.n), inner = static_cast<uint32_t>(p.inner);
+    const uint32_t t = fast_div(w, p.div_inner);
+    const uint32_t i = w - t * inner;
+    const uint32_t o = fast_div(t, p.div_runs);
+    const uint32_t j0 = (t - o * static_cast<uint32_t>(p.runs)) * R;
+    const uint32_t rows = min(static_cast<uint32_t>(R), n - j0);
+    const uint32_t base = o * n * inner + i;
+    const TI* __restrict__ x = static_cast<const TI*>(p.x);
+    // window entry q holds element j0 - W + q; entries past the last row's
+    // stencil, and out-of-range columns of a clamped dimension (whose
+    // matrix entries are zero), read element j0 and count as zero
+    double xv[R + 2 * W];
+#pragma unroll
+    for (int q = 0; q < R + 2 * W; ++q) {
+        int64_t jj = static_cast<int64_t>(j0) - W + q;
+        bool ok = static_cast<uint32_t>(q) < rows + 2 * W;
+        if (p.periodic) {
+            jj = jj < 0 ? jj + n : jj >= n ? jj - n : jj;
+        } else {
+            ok = ok && jj >= 0 && jj < n;
+        }
+        jj = ok ? jj : j0;
+        const double v = static_cast<double>(x[base + static_cast<uint32_t>(jj) * inner]);
+        xv[q] = ok ? v : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (static_cast<uint32_t>(r) >= rows) break;
+        const uint32_t j = j0 + r;
+        const uint32_t e = base + j * inner;
+        const double* __restrict__ row = p.colloc + j * static_cast<uint32_t>(2 * W + 1);
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m <= 2 * W; ++m) acc += row[m] * xv[r + m];
+        if (p.src) {
+            uint32_t rem = e;
+            uint64_t src = 0, mul = 1;
+            for (int d = p.dim - 1; d >= 0; --d) {
+                const uint32_t nd = static_cast<uint32_t>(p.dn[d]);
+                const uint32_t qd = fast_div(rem, p.div_dn[d]);
+                const uint32_t id = rem - qd * nd;
+                rem = qd;
+                int64_t s = static_cast<int64_t>(id) - p.dshift[d];
+                if (s < 0) s += nd;
+                src += static_cast<uint64_t>(s) * mul;
+                mul *= nd;
+            }
+            acc = p.src[src] - acc;
+        }
+        static_cast<TO*>(p.y)[e] = static_cast<TO>(acc);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void add_f32_kernel(float* __restrict__ c, const float* __restrict__ d,
                                                          int64_t total) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
     if (e < total) c[e] = c[e] + d[e];
 }
 
+constexpr int kRunRows = 8;
+
 template <int W>
 hipError_t launch_matvec_width(const MatvecParams& p, bool in_f32, bool out_f32, int64_t blocks, hipStream_t s) {
-    const dim3 grid(static_cast<unsigned>(blocks)), block(kBlock);
+    const dim3 block(kBlock);
+    if (p.runs > 0) {
+        const dim3 grid(static_cast<unsigned>((p.work + kBlock - 1) / kBlock));
+        if (in_f32 && out_f32)
+            hipLaunchKernelGGL((band_matvec_rows_kernel<W, kRunRows, float, float>), grid, block, 0, s, p);
+        else if (in_f32)
+            hipLaunchKernelGGL((band_matvec_rows_kernel<W, kRunRows, float, double>), grid, block, 0, s, p);
+        else if (out_f32)
+            hipLaunchKernelGGL((band_matvec_rows_kernel<W, kRunRows, double, float>), grid, block, 0, s, p);
+        else
+            hipLaunchKernelGGL((band_matvec_rows_kernel<W, kRunRows, double, double>), grid, block, 0, s, p);
+        return hipGetLastError();
+    }
+    const dim3 grid(static_cast<unsigned>(blocks));
     if (in_f32 && out_f32)
         hipLaunchKernelGGL((band_matvec_kernel<W, float, float>), grid, block, 0, s, p);
     else if (in_f32)
@@ -112,6 +193,14 @@ hipError_t launch_band_matvec(const MatvecParams& p_in, bool in_f32, bool out_f3
     p.xcd_remap = p.inner > 1 && (p.inner % kBlock != 0 || (p.inner / kBlock) % 8 != 0) ? 1 : 0;
     const int64_t blocks = (p.total + kBlock - 1) / kBlock;
     if (blocks == 0) return hipSuccess;
+    // axes before the last take the row-run kernel
+    p.runs = 0;
+    p.work = 0;
+    if (p.inner > 1 && p.n > 2 * static_cast<int64_t>(p.w)) {
+        p.runs = static_cast<int32_t>((p.n + kRunRows - 1) / kRunRows);
+        p.div_runs = make_fast_div(static_cast<uint32_t>(p.runs));
+        p.work = p.total / p.n * p.runs;
+    }
     switch (p.w) {
         case 0: return launch_matvec_width<0>(p, in_f32, out_f32, blocks, s);
         case 1: return launch_matvec_width<1>(p, in_f32, out_f32, blocks, s);
