@@ -62,7 +62,8 @@ def test_kernels_use_no_scratch():
     # the library's hot kernels are all there
     names = {_short(k) for k in kernels}
     for want in ("solve_tile_kernel", "solve_kernel", "eval_patch_kernel", "bin_scatter_kernel",
-                 "band_matvec_kernel", "batch_solve_kernel", "batch_line_split_kernel"):
+                 "band_matvec_kernel", "band_matvec_rows_kernel", "batch_solve_kernel", "batch_line_split_kernel",
+                 "scan_apply_kernel"):
         assert any(n.endswith(want) for n in names), want
     bad = []
     for k, md in kernels.items():
