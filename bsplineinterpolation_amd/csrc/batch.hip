@@ -51,6 +51,13 @@ constexpr int kMergedRows = 4;
 // four waves share the loads and stores of a large tile.
 constexpr int kSmallSolveThreads = 64;
 constexpr int64_t kSmallTileBytes = 20 * 1024;
+// Medium tiles (at most kMediumTileBytes: with the 1 KB of line bases,
+// four per CU) take two-wave workgroups: at two waves per SIMD four of
+// them are resident, so four waves per CU substitute instead of the two
+// of two 256-thread workgroups.  A launch allocates its largest item's
+// tile for every workgroup, hence a launch per size class.
+constexpr int kMediumSolveThreads = 128;
+constexpr int64_t kMediumTileBytes = 39 * 1024;
 constexpr int kLineThreads = 64;
 constexpr int kMaxBatchBand = 6;
 // 1D splines of at least this many points are split (INTP_BATCH_SPLIT_LINES)
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(kLineThreads) void batch_line_split_kernel(const Ba
 // items are ordered longest line first, so the long sequential lines
 // start at once and the short tiles fill in around them).
 template <int NT>
-__global__ __launch_bounds__(NT, NT == kSmallSolveThreads ? 2 : 1) void batch_solve_kernel(const BatchSpline* specs,
+__global__ __launch_bounds__(NT, NT == kTileThreads ? 1 : 2) void batch_solve_kernel(const BatchSpline* specs,
                                                                    const BatchSolveItem* items, int32_t nitems,
                                                                    const double* values, double* ctrl,
                                                                    int32_t first_pass) {
@@ -432,7 +439,7 @@ struct SolveGroup {
     int pass;
     int64_t offset, count;
     size_t lds;
-    int threads;  // kSmallSolveThreads or kTileThreads
+    int threads;  // kSmallSolveThreads, kMediumSolveThreads or kTileThreads
 };
 
 struct EvalGroup {
@@ -478,8 +485,9 @@ struct intp_batch_s {
     bool split_ready = false;
     hipStream_t side = nullptr;        // runs the 1D splines beside the passes
     hipStream_t side2 = nullptr;       // the split 1D lines, beside both
-    hipStream_t solve_side = nullptr;  // a pass's first solve group, beside its second
-    hipEvent_t solve_fork = nullptr, solve_join = nullptr;
+    // a pass's solve groups run side by side: all but its last on these
+    hipStream_t solve_side[2] = {};
+    hipEvent_t solve_fork = nullptr, solve_join[2] = {};
     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
     // the (dimension, order) eval classes run side by side on these
     hipStream_t eval_side[kEvalStreams] = {};
@@ -522,12 +530,14 @@ struct intp_batch_s {
             (void)hipStreamSynchronize(side2);
             (void)hipStreamDestroy(side2);
         }
-        if (solve_side) {
-            (void)hipStreamSynchronize(solve_side);
-            (void)hipStreamDestroy(solve_side);
+        for (int i = 0; i < 2; ++i) {
+            if (solve_side[i]) {
+                (void)hipStreamSynchronize(solve_side[i]);
+                (void)hipStreamDestroy(solve_side[i]);
+            }
+            if (solve_join[i]) (void)hipEventDestroy(solve_join[i]);
         }
         if (solve_fork) (void)hipEventDestroy(solve_fork);
-        if (solve_join) (void)hipEventDestroy(solve_join);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
         if (join2) (void)hipEventDestroy(join2);
@@ -569,13 +579,16 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
     auto kind_of = [](const DimSetup& ds, int mode) {
         return static_cast<int16_t>((mode << 4) | (2 * ds.lu.p + (ds.lu.cyclic ? 1 : 0)));
     };
-    // per pass: the large tiles and in-place runs (256-thread workgroups),
-    // and the small tiles (one-wave workgroups), each longest first
-    std::vector<std::pair<int64_t, BatchSolveItem>> pass_items, small_items;
+    // per pass: the large tiles with the in-place runs (256-thread
+    // workgroups), the medium tiles (two-wave workgroups) and the small
+    // tiles (one-wave workgroups), each longest first
+    std::vector<std::pair<int64_t, BatchSolveItem>> large_items, medium_items, small_items;
     for (int pass = 0; pass < kMaxDim; ++pass) {
-        SolveGroup g{pass, 0, 0, 0, kTileThreads};
+        SolveGroup gl{pass, 0, 0, 0, kTileThreads};
+        SolveGroup gm{pass, 0, 0, 0, kMediumSolveThreads};
         SolveGroup gs{pass, 0, 0, 0, kSmallSolveThreads};
-        pass_items.clear();
+        large_items.clear();
+        medium_items.clear();
         small_items.clear();
         for (int s = 0; s < b->count; ++s) {
             const int D = b->dims[static_cast<size_t>(s)];
@@ -593,20 +606,21 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
                 continue;  // batch_line_kernel, beside the passes
             } else if (inner >= 64) {
                 for (int64_t first = 0; first < nlines; first += kTileThreads)
-                    pass_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveLanes), first}});
+                    large_items.push_back(
+                        {cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveLanes), first}});
             } else {
                 const int64_t lpb = bs.lines_per_block[a];
                 const int64_t bytes = lpb * (ds.n | 1) * 8;
-                const bool small = bytes <= kSmallTileBytes;
+                const bool small = bytes <= kSmallTileBytes, medium = !small && bytes <= kMediumTileBytes;
+                auto& to_items = small ? small_items : medium ? medium_items : large_items;
                 for (int64_t first = 0; first < nlines; first += lpb)
-                    (small ? small_items : pass_items)
-                        .push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveTile), first}});
-                SolveGroup& to = small ? gs : g;
+                    to_items.push_back({cost, BatchSolveItem{s, static_cast<int16_t>(a), kind_of(ds, kSolveTile), first}});
+                SolveGroup& to = small ? gs : medium ? gm : gl;
                 to.lds = std::max(to.lds, static_cast<size_t>(bytes));
             }
         }
-        for (auto* items : {&pass_items, &small_items}) {
-            SolveGroup& grp = items == &pass_items ? g : gs;
+        for (auto* items : {&large_items, &medium_items, &small_items}) {
+            SolveGroup& grp = items == &large_items ? gl : items == &medium_items ? gm : gs;
             std::stable_sort(items->begin(), items->end(),
                              [](const auto& x, const auto& y) { return x.first > y.first; });
             grp.offset = static_cast<int64_t>(b->solve_host.size());
@@ -765,10 +779,12 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
         HIP_TRY(hipGetLastError(), "batch line launch");
         HIP_TRY(hipEventRecord(b->join, b->side), "hipEventRecord");
     }
-    if (!b->solve_side) {
-        HIP_TRY(hipStreamCreateWithFlags(&b->solve_side, hipStreamNonBlocking), "hipStreamCreate");
+    if (!b->solve_fork) {
+        for (int i = 0; i < 2; ++i) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->solve_side[i], hipStreamNonBlocking), "hipStreamCreate");
+            HIP_TRY(hipEventCreateWithFlags(&b->solve_join[i], hipEventDisableTiming), "hipEventCreate");
+        }
         HIP_TRY(hipEventCreateWithFlags(&b->solve_fork, hipEventDisableTiming), "hipEventCreate");
-        HIP_TRY(hipEventCreateWithFlags(&b->solve_join, hipEventDisableTiming), "hipEventCreate");
     }
     auto launch_group = [&](const SolveGroup& g, hipStream_t s) -> hipError_t {
         const int32_t first_pass = g.pass == 0 ? 1 : 0;
@@ -777,6 +793,12 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
             const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * 16));
             hipLaunchKernelGGL(batch_solve_kernel<kSmallSolveThreads>, dim3(static_cast<unsigned>(blocks)),
                                dim3(kSmallSolveThreads), g.lds, s, b->specs, b->solve_items + g.offset,
+                               static_cast<int32_t>(g.count), values, b->ctrl, first_pass);
+        } else if (g.threads == kMediumSolveThreads) {
+            // two waves each: 4 resident per CU (LDS and 2 waves per SIMD)
+            const int blocks = static_cast<int>(std::min<int64_t>(g.count, int64_t{b->num_cus} * 8));
+            hipLaunchKernelGGL(batch_solve_kernel<kMediumSolveThreads>, dim3(static_cast<unsigned>(blocks)),
+                               dim3(kMediumSolveThreads), g.lds, s, b->specs, b->solve_items + g.offset,
                                static_cast<int32_t>(g.count), values, b->ctrl, first_pass);
         } else {
             const int blocks =
@@ -787,21 +809,26 @@ int launch_build(intp_batch b, const double* values, hipStream_t stream, bool sp
         }
         return hipGetLastError();
     };
-    // a pass's two groups run side by side (they touch different splines);
-    // the next pass starts after both
-    for (size_t i = 0; i < b->solve_groups.size(); ++i) {
-        const SolveGroup& g = b->solve_groups[i];
-        const bool pair = i + 1 < b->solve_groups.size() && b->solve_groups[i + 1].pass == g.pass;
-        if (!pair) {
-            HIP_TRY(launch_group(g, stream), "batch solve launch");
-            continue;
+    // a pass's groups run side by side (a spline's lines along one axis
+    // are all in one size class, so the groups touch different splines):
+    // all but the last on the side streams; the next pass starts after all
+    // of them
+    for (size_t i = 0; i < b->solve_groups.size();) {
+        size_t end = i + 1;
+        while (end < b->solve_groups.size() && b->solve_groups[end].pass == b->solve_groups[i].pass) ++end;
+        const size_t nside = end - i - 1;  // at most 2: three size classes per pass
+        if (nside > 2) return fail(INTP_E_INVALID, "more solve groups in a pass than side streams");
+        if (nside > 0) {
+            HIP_TRY(hipEventRecord(b->solve_fork, stream), "hipEventRecord");
+            for (size_t k = 0; k < nside; ++k) {
+                HIP_TRY(hipStreamWaitEvent(b->solve_side[k], b->solve_fork, 0), "hipStreamWaitEvent");
+                HIP_TRY(launch_group(b->solve_groups[i + k], b->solve_side[k]), "batch solve launch");
+                HIP_TRY(hipEventRecord(b->solve_join[k], b->solve_side[k]), "hipEventRecord");
+            }
         }
-        HIP_TRY(hipEventRecord(b->solve_fork, stream), "hipEventRecord");
-        HIP_TRY(hipStreamWaitEvent(b->solve_side, b->solve_fork, 0), "hipStreamWaitEvent");
-        HIP_TRY(launch_group(g, b->solve_side), "batch solve launch");
-        HIP_TRY(hipEventRecord(b->solve_join, b->solve_side), "hipEventRecord");
-        HIP_TRY(launch_group(b->solve_groups[++i], stream), "batch solve launch");
-        HIP_TRY(hipStreamWaitEvent(stream, b->solve_join, 0), "hipStreamWaitEvent");
+        HIP_TRY(launch_group(b->solve_groups[end - 1], stream), "batch solve launch");
+        for (size_t k = 0; k < nside; ++k) HIP_TRY(hipStreamWaitEvent(stream, b->solve_join[k], 0), "hipStreamWaitEvent");
+        i = end;
     }
     if (lines) HIP_TRY(hipStreamWaitEvent(stream, b->join, 0), "hipStreamWaitEvent");
     if (lines && split && !b->split_host.empty()) HIP_TRY(hipStreamWaitEvent(stream, b->join2, 0), "hipStreamWaitEvent");
