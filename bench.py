@@ -491,8 +491,7 @@ def main():
                     help="copy 256 MiB once before timing (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU layout is described in DESIGN.md, "
-              "--gpus %d is not run" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch

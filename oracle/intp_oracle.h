@@ -1,7 +1,7 @@
 /*
  * intp_oracle.h -- CPU restatement of the reference B-spline interpolation
  * path (12ff54e/BSplineInterpolation), used ONLY as test infrastructure:
- * the checker in tests/, the reference leg of smoke(), and the cpu_baseline
+ * the tests in tests/, the reference leg of smoke(), and the cpu_baseline
  * leg of bench.py.  Nothing in the product (bsplineinterpolation_amd/,
  * include/) links or calls it.
  *

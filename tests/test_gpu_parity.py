@@ -4,7 +4,7 @@ answers and with the CPU oracle.
 Tolerances (norm-wise relative L2, the reference's rel_err):
   * MMA vectors: the reference tests' own bars, 1e-14 (interpolation) and
     1e-15 (explicit-knot B-splines);
-  * against the oracle: 1e-12 (the FP64 north-star bar); control points
+  * against the oracle: 1e-12 (the FP64 form's parity bar); control points
     are compared bit for bit (the GPU solve keeps the reference's operation
     order without FMA contraction and divides by the pivot).
 """

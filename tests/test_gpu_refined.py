@@ -1,7 +1,7 @@
 """Refined-FP32 form (intp_build_refined): FP32 control points solved with
 the FP32-rounded LU factors plus FP64-residual refinement sweeps, evaluated
-with FP32 weights and tables.  Checked against the FP64 oracle with the
-errors the north star asks for:
+with FP32 weights and tables.  Checked against the FP64 oracle with these
+error measures (at most 1e-6):
 
     max_rel = max|got - want| / max|want|,   rms_rel = rms(got - want) / rms(want)
 
