@@ -170,3 +170,33 @@ def test_gpu_batch_split_lines(amd, orc, order, periodic):
     o = orc.OracleSpline(order, f0, periodic=[periodic], ranges=[(0.0, 1.0)])
     want = o.eval(np.ascontiguousarray(pts[:q, :1]))[0]
     assert rel_err(split[:q].cpu().numpy(), want) < 1e-12
+
+
+def test_gpu_batch_tile_size_classes(amd):
+    """The tiles of a pass are launched in three workgroup shapes by their
+    LDS size (at most 20 KB: one wave; at most 39 KB: two waves; larger:
+    256 threads, with the in-place runs).  Splines whose lines land in
+    each class on every pass, periodic and clamped, band widths 1 to 4,
+    give control points and values bit-identical to the single-spline
+    path."""
+    import torch
+    rng = np.random.default_rng(33)
+    # 64 lines per tile unless a line is long: 64 * (n | 1) * 8 bytes
+    specs = [dict(order=3, shape=(40, 16), periodic=[False, False], ranges=[(0.0, 1.0)] * 2),    # small
+             dict(order=5, shape=(30, 50), periodic=[True, False], ranges=[(0.0, 1.0)] * 2),     # medium
+             dict(order=2, shape=(70, 150), periodic=[False, True], ranges=[(0.0, 1.0)] * 2),    # large
+             dict(order=4, shape=(20, 255), periodic=[False, False], ranges=[(0.0, 1.0)] * 2),   # large, 32 lines
+             dict(order=3, shape=(50, 30, 60), periodic=[False, True, False], ranges=[(0.0, 1.0)] * 3),
+             dict(order=1, shape=(100, 9, 12), periodic=[True, False, False], ranges=[(0.0, 1.0)] * 3)]
+    b = amd.Batch(specs)
+    vals = rng.standard_normal(b.total)
+    q = 200
+    pts = rng.random((q * len(specs), 3)) * 0.999
+    po = np.arange(len(specs) + 1) * q
+    out = b.run(torch.from_numpy(vals).cuda(), torch.from_numpy(pts).cuda(), po).cpu().numpy()
+    for s, sp in enumerate(specs):
+        f = vals[b.offsets[s]:b.offsets[s + 1]].reshape(sp["shape"])
+        g = amd.InterpolationFunction(sp["order"], f, periodic=sp["periodic"], ranges=sp["ranges"])
+        assert np.array_equal(b.control_points(s), g.control_points().ravel()), s
+        D = len(sp["shape"])
+        assert np.array_equal(out[po[s]:po[s + 1]], g.eval(pts[po[s]:po[s + 1], :D])[0]), s
