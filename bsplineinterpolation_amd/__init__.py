@@ -41,6 +41,24 @@ def _current_stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _keep_for_stream(tmp, orig, stream):
+    """A temporary made by .contiguous() on the current stream is freed
+    when the call returns; if the library reads it on another stream, the
+    caching allocator must not hand the block out before that stream is
+    done with it."""
+    if stream is None or tmp is orig or not hasattr(stream, "cuda_stream"):
+        return
+    tmp.record_stream(stream)
+
+
+def _check_device_f64(t, what, numel=None):
+    import torch
+    if not _is_torch(t) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous float64 GPU tensor" % what)
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s must hold %d values, got %d" % (what, numel, t.numel()))
+
+
 class Context:
     """Binds one GPU (intp_ctx_create)."""
 
@@ -112,9 +130,12 @@ class InterpolationFunction:
         h = C.c_void_p()
         refined = _refined(precision)
         if _is_torch(f):
+            f0 = f
             f = f.contiguous()
-            assert str(f.dtype) == "torch.float64", "values must be float64"
+            if str(f.dtype) != "torch.float64" or not f.is_cuda:
+                raise ValueError("device values must be a float64 GPU tensor")
             st = _stream_handle(stream) if stream is not None else _current_stream()
+            _keep_for_stream(f, f0, stream)
             vp, mem = C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE
         else:
             fa = np.ascontiguousarray(f, dtype=np.float64)
@@ -215,21 +236,29 @@ class InterpolationFunction:
         op = orders.ctypes.data_as(C.POINTER(C.c_int32))
         if _is_torch(points):
             import torch
-            assert points.dtype == torch.float64 and points.is_cuda
+            if points.dtype != torch.float64 or not points.is_cuda:
+                raise ValueError("device points must be a float64 GPU tensor")
             pts = points.contiguous()
             if pts.dim() == 1 and D == 1:
                 pts = pts.reshape(1, -1)
-            if pts.shape[0] == D and (pts.dim() == 2 and (pts.shape[1] != D or D == 1)):
+            if pts.dim() != 2 or D not in tuple(pts.shape):
+                raise ValueError("points must be [%d, npts] (SoA) or [npts, %d] (AoS), got %s"
+                                 % (D, D, tuple(pts.shape)))
+            if pts.shape[0] == D and (pts.shape[1] != D or D == 1):
                 npts, cs_dim, cs_pt = pts.shape[1], pts.shape[1], 1
             else:
                 npts, cs_dim, cs_pt = pts.shape[0], 1, D
             if out is None:
                 dt = torch.float32 if out_dtype in (torch.float32, np.float32, "float32") else torch.float64
                 out = torch.empty((nout, npts), dtype=dt, device=pts.device)
-            assert tuple(out.shape) == (nout, npts) and out.dtype in (torch.float64, torch.float32) and out.is_cuda
+            if (tuple(out.shape) != (nout, npts) or out.dtype not in (torch.float64, torch.float32)
+                    or not out.is_cuda or out.device != pts.device):
+                raise ValueError("out must be a float64/float32 tensor [%d, %d] on the points' device"
+                                 % (nout, npts))
             if out.dtype == torch.float32:
                 flags |= _lib.INTP_EVAL_OUT_F32
             st = _stream_handle(stream) if stream is not None else _current_stream()
+            _keep_for_stream(pts, points, stream)
             check(lib.intp_eval(self.h, npts, C.cast(pts.data_ptr(), _dp), cs_dim, cs_pt, nout, op,
                                 C.cast(out.data_ptr(), _dp), out.stride(0), out.stride(1), flags, st))
             return out
@@ -318,8 +347,12 @@ class InterpolationFunctionTemplate:
             raise IntpError(_lib.INTP_E_INVALID, "value mesh shape does not match the template")
         h = C.c_void_p()
         if _is_torch(f):
+            f0 = f
             f = f.contiguous()
+            if str(f.dtype) != "torch.float64" or not f.is_cuda:
+                raise ValueError("device values must be a float64 GPU tensor")
             st = _stream_handle(stream) if stream is not None else _current_stream()
+            _keep_for_stream(f, f0, stream)
             vp, mem = C.cast(f.data_ptr(), _dp), _lib.INTP_MEM_DEVICE
         else:
             fa = np.ascontiguousarray(f, dtype=np.float64)
@@ -365,20 +398,34 @@ class Batch:
             _lib._LIB.intp_batch_destroy(self.h)
             self.h = None
 
-    @staticmethod
-    def _offsets(point_offsets):
+    def _offsets(self, point_offsets):
         po = np.ascontiguousarray(np.asarray(point_offsets, dtype=np.int64))
+        if po.ndim != 1 or po.size != self.count + 1:
+            raise ValueError("point_offsets must hold count + 1 = %d entries" % (self.count + 1))
+        if po[0] != 0 or np.any(np.diff(po) < 0):
+            raise ValueError("point_offsets must start at 0 and be non-decreasing")
         return po, po.ctypes.data_as(C.POINTER(C.c_int64))
 
+    def _points(self, coords, po, out):
+        import torch
+        _check_device_f64(coords, "coords")
+        if coords.dim() != 2 or coords.shape[1] != 3 or coords.shape[0] < int(po[-1]):
+            raise ValueError("coords must be [npts, 3] with npts >= point_offsets[-1] = %d" % int(po[-1]))
+        if out is None:
+            out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
+        _check_device_f64(out, "out")
+        if out.numel() < int(po[-1]):
+            raise ValueError("out must hold point_offsets[-1] = %d values" % int(po[-1]))
+        return out
+
     def build(self, values, stream=None):
+        _check_device_f64(values, "values", self.total)
         st = _stream_handle(stream) if stream is not None else _current_stream()
         check(_lib.load().intp_batch_build(self.h, C.cast(values.data_ptr(), _dp), st))
 
     def eval(self, coords, point_offsets, out=None, stream=None):
-        import torch
         po, pp = self._offsets(point_offsets)
-        if out is None:
-            out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
+        out = self._points(coords, po, out)
         st = _stream_handle(stream) if stream is not None else _current_stream()
         check(_lib.load().intp_batch_eval(self.h, C.cast(coords.data_ptr(), _dp), pp, C.cast(out.data_ptr(), _dp), st))
         return out
@@ -389,10 +436,9 @@ class Batch:
         hipGraph while buffers and layout stay the same; ``split_lines``
         solves long 1D lines in parallel segments (INTP_BATCH_SPLIT_LINES:
         within 1e-12 of the sequential solve, not bit-identical)."""
-        import torch
+        _check_device_f64(values, "values", self.total)
         po, pp = self._offsets(point_offsets)
-        if out is None:
-            out = torch.empty(int(po[-1]), dtype=torch.float64, device=coords.device)
+        out = self._points(coords, po, out)
         flags = ((_lib.INTP_BATCH_GRAPH if graph else 0) | (_lib.INTP_BATCH_AUTOTUNE if autotune else 0) |
                  (_lib.INTP_BATCH_SPLIT_LINES if split_lines else 0))
         st = _stream_handle(stream) if stream is not None else _current_stream()

@@ -555,8 +555,12 @@ struct intp_batch_s {
 
 namespace {
 
+void drop_graph(intp_batch b);
+
 // (Re)builds the solve work items for the current tuning; uploads them on
-// `stream` (the host copy stays alive in the batch).
+// `stream` (the host copy stays alive in the batch).  A captured graph
+// bakes in the work-item buffers and their contents, so it is dropped,
+// and the upload waits for every launch still reading the old items.
 int make_solve_work(intp_batch b, hipStream_t stream) {
     // per spline tile sizes (tiles serve the axes with fewer than 64
     // adjacent lines)
@@ -629,6 +633,8 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
             if (grp.count > 0) b->solve_groups.push_back(grp);
         }
     }
+    drop_graph(b);
+    b->uses.order_before(stream);
     const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->solve_host.size()));
     if (need > b->solve_cap) {
         release_after(b->device, b->solve_items, b->uses);
@@ -652,6 +658,7 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
 int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t stream) {
     const std::vector<int64_t> layout(point_offsets, point_offsets + b->count + 1);
     if (layout == b->eval_layout && b->eval_threads_used == b->tune.eval_threads) return INTP_OK;
+    if (layout[0] < 0) return fail(INTP_E_INVALID, "point offsets must start at 0 or more");
     for (int s = 0; s < b->count; ++s)
         if (layout[static_cast<size_t>(s) + 1] < layout[static_cast<size_t>(s)])
             return fail(INTP_E_INVALID, "point offsets must be non-decreasing");
@@ -670,6 +677,9 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
             g.count = static_cast<int64_t>(b->eval_host.size()) - g.offset;
             if (g.count > 0) b->eval_groups.push_back(g);
         }
+    // a captured graph reads the old items (and maybe the old buffer)
+    drop_graph(b);
+    b->uses.order_before(stream);
     const int64_t need = std::max<int64_t>(1, static_cast<int64_t>(b->eval_host.size()));
     if (need > b->eval_cap) {
         release_after(b->device, b->eval_items, b->uses);

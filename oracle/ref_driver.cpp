@@ -168,6 +168,7 @@ void* ref_build(int dim, int order, const int* per, const int64_t* n,
             case 4: return build_k<4>(dim, per, n, lo, hi, xc, f);
             case 5: return build_k<5>(dim, per, n, lo, hi, xc, f);
             case 6: return build_k<6>(dim, per, n, lo, hi, xc, f);
+            case 7: return build_k<7>(dim, per, n, lo, hi, xc, f);
             default: throw std::invalid_argument("order not instantiated");
         }
     } catch (const std::exception& e) {

@@ -80,6 +80,55 @@ def test_gpu_batch_graph_replay_and_autotune(amd, work):
     print(rep)
 
 
+def test_gpu_batch_graph_after_layout_change(amd, work):
+    """A captured graph bakes in the eval work items.  An evaluation with
+    a larger point layout reallocates them; a graph run with the first
+    layout afterwards must not replay the stale graph, and equals the plain
+    run (twice, so the recaptured graph is replayed too)."""
+    import torch
+    b, vals, pts = _setup(amd, work)
+    po = work["point_offsets"]
+    stream = torch.cuda.Stream()
+    big_po = po * 4
+    big_pts = pts.repeat(4, 1).contiguous()
+    with torch.cuda.stream(stream):
+        plain = b.run(vals, pts, po).clone()
+        out = torch.empty_like(plain)
+        b.run(vals, pts, po, out=out, graph=True)
+        big = b.eval(big_pts, big_po)
+        out.zero_()
+        b.run(vals, pts, po, out=out, graph=True)
+        again = out.clone()
+        out.zero_()
+        b.run(vals, pts, po, out=out, graph=True)
+    stream.synchronize()
+    assert torch.equal(again, plain) and torch.equal(out, plain)
+    assert big.numel() == int(big_po[-1])
+
+
+def test_gpu_batch_rejects_bad_buffers(amd, work):
+    """Buffers that do not match the batch raise before any launch."""
+    import torch
+    b, vals, pts = _setup(amd, work)
+    po = work["point_offsets"]
+    with pytest.raises(ValueError):
+        b.build(vals[:-1].contiguous())
+    with pytest.raises(ValueError):
+        b.build(vals.float())
+    with pytest.raises(ValueError):
+        b.eval(pts[:10].contiguous(), po)
+    with pytest.raises(ValueError):
+        b.eval(pts[:, :2].contiguous(), po)
+    with pytest.raises(ValueError):
+        b.eval(pts, po[:-1])
+    bad = po.copy()
+    bad[0] = -1
+    with pytest.raises(ValueError):
+        b.eval(pts, bad)
+    with pytest.raises(ValueError):
+        b.run(vals, pts, po, out=torch.empty(3, dtype=torch.float64, device="cuda"))
+
+
 def test_gpu_batch_separate_build_and_eval(amd, work):
     import torch
     b, vals, pts = _setup(amd, work)

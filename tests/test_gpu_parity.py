@@ -201,3 +201,22 @@ def test_gpu_solve_tiling_and_long_lines(amd, orc, shape, k, per):
     o = orc.OracleSpline(k, f, periodic=per, ranges=rg)
     g = amd.InterpolationFunction(k, f, periodic=per, ranges=rg)
     assert np.array_equal(g.control_points(), o.control_points())
+
+
+def test_gpu_device_points_shape_checked(amd):
+    """Device point tensors whose shape does not match the spline's
+    dimension raise before any launch."""
+    import torch
+    rng = np.random.default_rng(8)
+    g = amd.InterpolationFunction(3, rng.standard_normal((8, 9, 10)), ranges=[(0, 1)] * 3)
+    with pytest.raises(ValueError):
+        g.eval(torch.zeros((50, 2), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        g.eval(torch.zeros(3, dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        g.eval(torch.zeros((50, 3), dtype=torch.float32, device="cuda"))
+    with pytest.raises(ValueError):
+        g.eval(torch.zeros((50, 3), dtype=torch.float64, device="cuda"),
+               out=torch.empty((1, 49), dtype=torch.float64, device="cuda"))
+    ok = g.eval(torch.full((50, 3), 0.5, dtype=torch.float64, device="cuda"))
+    assert ok.shape == (1, 50)

@@ -115,7 +115,7 @@ def test_oracle_uniform_knots_are_not_a_knot(orc, k):
     assert dx == pytest.approx(2.0 / (n - 1), rel=1e-15)
 
 
-@pytest.mark.parametrize("k", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_oracle_periodic_uniform_knots(orc, k):
     """Periodic: equally spaced with spacing P/n, data sites at knots (odd k)
     or at knot midpoints (even k); range covers exactly one period."""
@@ -146,7 +146,7 @@ def test_oracle_nonuniform_knots_moving_average(orc):
 # basis, span
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("k", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_oracle_basis_matches_scipy(orc, k):
     """Cox-de Boor against scipy's design matrix on a non-uniform clamped
     knot vector; also partition of unity."""
@@ -212,7 +212,7 @@ def test_oracle_span_search_brute_force(orc):
 # end-to-end interpolation against scipy / linear algebra
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
 def test_oracle_clamped_1d_matches_make_interp_spline(orc, k):
     """Same knots, same data sites -> the interpolant is unique, so scipy's
     make_interp_spline(t=...) must give the same coefficients."""
@@ -230,7 +230,7 @@ def test_oracle_clamped_1d_matches_make_interp_spline(orc, k):
         assert np.allclose(s.eval(q[:, None], [[r]])[0], sp(q, nu=r), rtol=1e-10, atol=1e-10)
 
 
-@pytest.mark.parametrize("k", [1, 3, 5])
+@pytest.mark.parametrize("k", [1, 3, 5, 7])
 def test_oracle_periodic_odd_matches_scipy(orc, k):
     """Odd order, periodic, uniform: scipy's periodic knots coincide with the
     reference's, so values must agree."""
@@ -267,7 +267,7 @@ def test_oracle_periodic_even_dense_solve(orc, k):
     assert np.allclose(s.eval(x[:, None])[0], f, rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5, 6, 7])
 def test_oracle_polynomial_reproduction_3d(orc, k):
     """Clamped data sampled from a polynomial of degree <= k is reproduced
     exactly (to round-off), values and partial derivatives."""

@@ -21,13 +21,13 @@ def _configs():
     out = []
     rng = np.random.default_rng(2024)
     for D in (1, 2, 3):
-        for k in range(0, 7):
+        for k in range(0, 8):
             for per in itertools.product([False, True], repeat=D):
-                if D == 3 and k not in (2, 3, 5):
+                if D == 3 and k not in (2, 3, 5, 7):
                     continue
                 n = [int(rng.integers(max(k + 2, 6), 14)) for _ in range(D)]
                 out.append((D, k, list(per), n, False))
-    for D, k in ((1, 1), (1, 2), (1, 3), (1, 4), (1, 5), (2, 3), (2, 4)):
+    for D, k in ((1, 1), (1, 2), (1, 3), (1, 4), (1, 5), (1, 6), (1, 7), (2, 3), (2, 4), (2, 7)):
         for per in itertools.product([False, True], repeat=D):
             n = [int(rng.integers(k + 3, 12)) for _ in range(D)]
             out.append((D, k, list(per), n, True))
