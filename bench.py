@@ -55,6 +55,28 @@ def profiled_traffic(kernel: str):
     return None
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _compiler() -> str:
+    """The compiler that built the CPU baseline library (g++ here), as
+    `g++ --version` reports it."""
+    import subprocess
+    try:
+        return subprocess.run(["g++", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        return "unknown"
+
+
 def _cores() -> int:
     n = os.cpu_count() or 1
     for var in ("OMP_NUM_THREADS",):
@@ -284,7 +306,8 @@ def measure_batched(amd, torch, steps: int):
             "split_lines": {"run_ms": split_ms, "points_per_s": Q / (split_ms * 1e-3),
                             "gbs": (b_solve + b_eval) / (split_ms * 1e-3) / 1e9,
                             "max_rel_diff_vs_sequential": d},
-            "parity": "control points and values bit-identical to the single-spline path (tests/test_gpu_batch.py)"}
+            "parity": "control points and values of all 4096 splines bit-identical to the single-spline path, "
+                      "autotuned and graph-replayed (tests/test_gpu_configs.py::test_gpu_config3_batch_4096_mixed)"}
 
 
 def measure_serving(amd, torch, seconds: float, rates, burst: int, batch_gbs=None):
@@ -461,7 +484,9 @@ def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
     nq = w["nq"]
     step_s = t_build + t_mt * nq
     errs = [rel_stats(gpu_out_sample[i], ref[i]) for i in range(len(w["orders"]))]
-    return dict(value=nq / step_s, unit="points/s", cores=cores, kind=kind,
+    return dict(value=nq / step_s, unit="points/s", cores=cores, kind=kind, cpu_model=_cpu_model(),
+                compiler=_compiler() + " -O3 -DNDEBUG -DINTP_MULTITHREAD -DINTP_PERIODIC_NO_DUMMY_POINT "
+                                       "-DINTP_CELL_LAYOUT (oracle/Makefile)",
                 sample=("constructor at full size (%.3f s, 8 pool workers + caller); evaluation of value + 3 "
                         "partials on %d of the 2^26 queries with %d threads (%.3f us/point), scaled to 2^26; "
                         "single-thread %.3f us/point on %d points" % (t_build, sample_mt, cores, t_mt * 1e6,
@@ -601,7 +626,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded SplitMix64/Box-Muller generator, bsplineinterpolation_amd/datagen.py)",
         "config": {"workload": "3D %d^3 order-3 FP64 build (dim 0 periodic, dims 1-2 clamped) + %d queries x "
-                               "(value, d/dx, d/dy, d/dz)" % (n, nq),
+                               "(value, d/dx, d/dy, d/dz); each step runs the full constructor (control-point "
+                               "solve from the values) while the grid's knots and LU factors come from the "
+                               "library's cache of recent grids after the first step" % (n, nq),
                    "mesh": [n, n, n], "order": 3, "periodic": per, "queries": nq, "outputs": 4},
         "breakdown_ms": {"build_full_constructor": build_ms, "solve_only": solve_ms, "eval_call": eval_ms,
                          "eval_grouping": group_ms, "eval_kernel": kernel_ms},

@@ -18,8 +18,8 @@ import numpy as np
 from . import _lib
 from ._lib import DomainError, IntpError, check  # noqa: F401
 
-__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "Batch", "Server", "DomainError",
-           "IntpError", "eval_timings"]
+__all__ = ["Context", "InterpolationFunction", "InterpolationFunctionTemplate", "EvalProxy", "Batch", "Server",
+           "DomainError", "IntpError", "eval_timings"]
 
 _dp = C.POINTER(C.c_double)
 
@@ -213,6 +213,28 @@ class InterpolationFunction:
         return out
 
     # ---- evaluation ---------------------------------------------------
+    def eval_host(self, points, orders=None, bounds: bool = False):
+        """Evaluation on this thread from a host copy of the control points
+        (INTP_EVAL_HOST_MIRROR; the copy is made on the first call): the
+        per-point path of the C++ drop-in's scalar members.  ``points``:
+        numpy [npts, D]; returns numpy [nout, npts] float64."""
+        D = self.dim
+        orders = np.zeros((1, D), np.int32) if orders is None else \
+            np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, D))
+        res = np.empty((orders.shape[0], pts.shape[0]))
+        flags = _lib.INTP_EVAL_HOST_MIRROR | (_lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0)
+        check(_lib.load().intp_eval(self.h, pts.shape[0], pts.ctypes.data_as(_dp), 1, D, orders.shape[0],
+                                    orders.ctypes.data_as(C.POINTER(C.c_int32)), res.ctypes.data_as(_dp),
+                                    pts.shape[0], 1, flags, None))
+        return res
+
+    def eval_proxy(self, points, bounds: bool = False, stream=None) -> "EvalProxy":
+        """eval_proxy (Interpolation.hpp:493-506) for many points: spans and
+        basis values of ``points`` on this spline's grid, computed once;
+        the returned proxy evaluates any spline on an equal grid."""
+        return EvalProxy(points, spline=self, bounds=bounds, stream=stream)
+
     def eval(self, points, orders=None, bounds: bool = False, out=None, stream=None, timed: bool = False,
              out_dtype=None, group: Optional[bool] = None):
         """Batched evaluation.  ``points``: numpy [npts, D] (host) or a
@@ -222,8 +244,9 @@ class InterpolationFunction:
         float32 when ``out`` is float32 or ``out_dtype`` says so.  ``timed``
         records device events around the call (read with eval_timings()).
         ``group``: True / False forces / forbids grouping the points by
-        brick (INTP_EVAL_GROUPED / INTP_EVAL_UNGROUPED); None lets the
-        library decide from the batch and table sizes.  Results are the
+        brick (INTP_EVAL_GROUPED / INTP_EVAL_UNGROUPED); "two_pass" forces
+        the two-radix-pass grouping (INTP_EVAL_GROUP_TWO_PASS); None lets
+        the library decide from the batch and table sizes.  Results are the
         same either way."""
         lib = _lib.load()
         D = self.dim
@@ -231,7 +254,9 @@ class InterpolationFunction:
             np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
         nout = orders.shape[0]
         flags = (_lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0) | (_lib.INTP_EVAL_TIMED if timed else 0)
-        if group is not None:
+        if group == "two_pass":
+            flags |= _lib.INTP_EVAL_GROUPED | _lib.INTP_EVAL_GROUP_TWO_PASS
+        elif group is not None:
             flags |= _lib.INTP_EVAL_GROUPED if group else _lib.INTP_EVAL_UNGROUPED
         op = orders.ctypes.data_as(C.POINTER(C.c_int32))
         if _is_torch(points):
@@ -363,6 +388,71 @@ class InterpolationFunctionTemplate:
         else:
             check(lib.intp_interpolate(self.h, vp, mem, C.byref(h), st))
         return InterpolationFunction._wrap(h, self.ctx)
+
+    def eval_proxy(self, points, bounds: bool = False, stream=None) -> EvalProxy:
+        """eval_proxy (InterpolationTemplate.hpp:145-176) for many points."""
+        return EvalProxy(points, template=self, bounds=bounds, stream=stream)
+
+
+class EvalProxy:
+    """Evaluation proxy (intp_proxy_*; the reference's eval_proxy /
+    pre_calc_coef, Interpolation.hpp:493-506, BSpline.hpp:244-297): the knot
+    spans and value basis of fixed ``points`` on a grid (of ``template`` or
+    of ``spline``), computed once.  ``proxy(f)`` returns the values of any
+    spline ``f`` on an equal grid at those points.  numpy points make a
+    host proxy (applied on this thread to the spline's host copy of its
+    control points); a float64 torch tensor [npts, D] makes a device proxy
+    whose values equal ``f.eval`` bit for bit."""
+
+    def __init__(self, points, template=None, spline=None, bounds: bool = False, stream=None):
+        lib = _lib.load()
+        if (template is None) == (spline is None):
+            raise ValueError("give exactly one of template and spline")
+        owner = template if template is not None else spline
+        D = owner.dim if spline is not None else len(owner.shape)
+        self._owner = owner
+        flags = _lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0
+        self.host = not _is_torch(points)
+        if self.host:
+            pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, D))
+            ptr, st = pts.ctypes.data_as(_dp), None
+            flags |= _lib.INTP_EVAL_HOST_MEM
+        else:
+            import torch
+            if points.dtype != torch.float64 or not points.is_cuda:
+                raise ValueError("device points must be a float64 GPU tensor")
+            pts = points.contiguous().reshape(-1, D)
+            ptr = C.cast(pts.data_ptr(), _dp)
+            st = _stream_handle(stream) if stream is not None else _current_stream()
+        self.npts = int(pts.shape[0])
+        self.device = None if self.host else pts.device
+        h = C.c_void_p()
+        check(lib.intp_proxy_create(template.h if template is not None else None,
+                                    spline.h if spline is not None else None, self.npts, ptr, 1, D, flags,
+                                    C.byref(h), st))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None and _lib._LIB is not None:
+            _lib._LIB.intp_proxy_destroy(self.h)
+            self.h = None
+
+    def __call__(self, f: "InterpolationFunction", out=None, stream=None):
+        lib = _lib.load()
+        if self.host:
+            res = np.empty(self.npts)
+            check(lib.intp_proxy_eval(self.h, f.h, res.ctypes.data_as(_dp), 1, 0, None))
+            return res
+        import torch
+        if out is None:
+            out = torch.empty(self.npts, dtype=torch.float64, device=self.device)
+        if out.numel() != self.npts or not out.is_cuda or out.dtype not in (torch.float64, torch.float32):
+            raise ValueError("out must be a float64/float32 GPU tensor of %d values" % self.npts)
+        flags = _lib.INTP_EVAL_OUT_F32 if out.dtype == torch.float32 else 0
+        st = _stream_handle(stream) if stream is not None else _current_stream()
+        check(lib.intp_proxy_eval(self.h, f.h, C.cast(out.data_ptr(), _dp), out.stride(0) if out.dim() else 1,
+                                  flags, st))
+        return out
 
 
 class Batch:

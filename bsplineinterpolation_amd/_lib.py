@@ -30,6 +30,8 @@ INTP_EVAL_UNGROUPED = 4
 INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
 INTP_EVAL_GROUPED = 32
+INTP_EVAL_HOST_MIRROR = 64
+INTP_EVAL_GROUP_TWO_PASS = 128
 INTP_BATCH_GRAPH = 1
 INTP_BATCH_AUTOTUNE = 2
 INTP_BATCH_SPLIT_LINES = 4
@@ -53,6 +55,7 @@ EXPORTS = (
     "intp_batch_run", "intp_batch_tuning", "intp_batch_download_ctrl",
     "intp_server_create", "intp_server_destroy", "intp_serve_build", "intp_serve_eval", "intp_serve_poll",
     "intp_serve_wait", "intp_serve_drain", "intp_serve_release", "intp_serve_spline", "intp_serve_stats",
+    "intp_proxy_create", "intp_proxy_eval", "intp_proxy_points", "intp_proxy_destroy",
 )
 
 
@@ -146,6 +149,10 @@ def load(path: str = LIB_PATH):
     lib.intp_serve_release.argtypes = [vp, C.c_int64]
     lib.intp_serve_spline.argtypes = [vp, C.c_int64, C.POINTER(vp)]
     lib.intp_serve_stats.argtypes = [vp, C.c_int32, C.POINTER(ServeCounters)]
+    lib.intp_proxy_create.argtypes = [vp, vp, C.c_int64, dp, C.c_int64, C.c_int64, C.c_uint32, C.POINTER(vp), vp]
+    lib.intp_proxy_eval.argtypes = [vp, vp, dp, C.c_int64, C.c_uint32, vp]
+    lib.intp_proxy_points.argtypes = [vp, i64p]
+    lib.intp_proxy_destroy.argtypes = [vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

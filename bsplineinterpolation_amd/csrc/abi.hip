@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstddef>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <list>
@@ -19,6 +20,7 @@
 #include "../../include/intp_amd.h"
 #include "kernels.hpp"
 #include "grid.hpp"
+#include "host_eval.hpp"
 #include "runtime.hpp"
 #include "setup.hpp"
 
@@ -175,6 +177,12 @@ struct intp_spline_s {
     int ctrl_bytes = 8;
     int sweeps = -1;       // refinement sweeps of a refined build
     UseTracker uses;       // streams that read `ctrl` (solve and every eval)
+    // Host copy of the control points (FP64), downloaded on the first
+    // host-side evaluation; immutable afterwards like the device table.
+    std::once_flag mirror_once;
+    int mirror_rc = INTP_OK;
+    std::string mirror_error;
+    std::vector<double> mirror;
     ~intp_spline_s() {
         // ordered on the release stream before the grid's own release,
         // which runs when `grid` drops its last reference below
@@ -520,6 +528,57 @@ BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
     return bp;
 }
 
+// Bricks of the one-pass grouping: per dimension a power of two, grown
+// round-robin from the last dimension while the (B_d + order) patch of
+// elem_bytes elements fits kBrickPatchBytes (a brick no longer than its
+// dimension, and a last-dimension patch row no longer than the kernel's
+// workgroup).  False when that leaves more than kGroupMaxBins bricks.
+bool item_bricks(const GridData& g, int elem_bytes, BrickGrid* out, int64_t* patch_volume, int32_t* nbins) {
+    int shift[kMaxDim] = {0, 0, 0};
+    auto volume = [&]() {
+        int64_t v = 1;
+        for (int d = 0; d < g.dim; ++d) v *= (int64_t{1} << shift[d]) + g.order;
+        return v;
+    };
+    for (bool grew = true; grew;) {
+        grew = false;
+        for (int d = g.dim - 1; d >= 0; --d) {
+            if ((int64_t{1} << (shift[d] + 1)) > g.setup[d].n) continue;
+            if (d == g.dim - 1 && (int64_t{1} << (shift[d] + 1)) + g.order > kBrickThreads) continue;
+            ++shift[d];
+            if (volume() * elem_bytes <= kBrickPatchBytes)
+                grew = true;
+            else
+                --shift[d];
+        }
+    }
+    int64_t bins = 1;
+    for (int d = 0; d < g.dim; ++d) {
+        out->brick_shift[d] = shift[d];
+        out->nbrick[d] = static_cast<int32_t>((g.setup[d].n + (int64_t{1} << shift[d]) - 1) >> shift[d]);
+        bins *= out->nbrick[d];
+    }
+    if (bins > kGroupMaxBins || volume() * elem_bytes > kBrickPatchBytes) return false;
+    *patch_volume = volume();
+    *nbins = static_cast<int32_t>(bins);
+    return true;
+}
+
+// Scratch bytes of the one-pass grouping (records, bin-major counts and
+// their scan, scan tiles, brick items, item count) and of the two-pass
+// grouping (two record arrays, histograms, keys).
+int64_t one_pass_scratch(int64_t npts, int32_t nbins) {
+    auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+    const int64_t nh = static_cast<int64_t>(nbins) * group_blocks(npts, nbins);
+    return up(npts * static_cast<int64_t>(sizeof(BinRecord))) + 2 * up(nh * 4) + up(scan_tile_sums(nh) * 4) +
+           up(group_max_items(npts, nbins) * static_cast<int64_t>(sizeof(BrickItem))) + 256;
+}
+
+int64_t two_pass_scratch(int64_t npts) {
+    const int64_t nh = bin_hist_entries(npts), nt = bin_scan_tiles(npts);
+    return 2 * ((npts * static_cast<int64_t>(sizeof(BinRecord)) + 255) / 256 * 256) + (2 * nh + nt + npts) * 4 + 256;
+}
+
 // The uniform fast path needs every dimension uniform (knots in closed
 // form, basis polynomial classes uploaded) and 32-bit table offsets.
 bool uniform_path(const GridData& g) {
@@ -553,6 +612,7 @@ int intp_ctx_create(int device, intp_ctx* out) {
     int count = 0;
     HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
     if (device < 0 || device >= count) return fail(INTP_E_INVALID, "no such device");
+    if (!device_runtime(device)) return fail(INTP_E_HIP, "cannot create the device's streams");
     *out = new intp_ctx_s{device};
     return INTP_OK;
 }
@@ -766,8 +826,106 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
     return INTP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The spline's control points on the host (FP64), downloaded once.
+int ensure_mirror(intp_spline s) {
+    std::call_once(s->mirror_once, [s]() {
+        std::vector<double> host(static_cast<size_t>(s->grid->total));
+        s->mirror_rc = intp_spline_download_ctrl(s, host.data(), s->grid->total);
+        if (s->mirror_rc == INTP_OK)
+            s->mirror = std::move(host);
+        else
+            s->mirror_error = g_error;
+    });
+    if (s->mirror_rc != INTP_OK) return fail(s->mirror_rc, s->mirror_error);
+    return INTP_OK;
+}
+
+HostSpline host_view(const intp_spline_s& s) {
+    HostSpline h{};
+    h.dim = s.grid->dim;
+    h.order = s.grid->order;
+    for (int d = 0; d < h.dim; ++d) h.dims[d] = &s.grid->setup[d];
+    h.ctrl = s.mirror.data();
+    return h;
+}
+
+int domain_error_at(const GridData& g, int64_t i, const double* x) {
+    for (int d = 0; d < g.dim; ++d) {
+        const DimSetup& ds = g.setup[d];
+        if (!ds.periodic && !(x[d] >= ds.lo && x[d] <= ds.hi)) {
+            char buf[256];
+            std::snprintf(buf, sizeof buf,
+                          "Given coordinate out of interpolation function range! (point %lld, dimension %d: %.17g "
+                          "outside [%.17g, %.17g])",
+                          static_cast<long long>(i), d, x[d], ds.lo, ds.hi);
+            return fail(INTP_E_DOMAIN, buf);
+        }
+    }
+    return fail(INTP_E_DOMAIN, "Given coordinate out of interpolation function range!");
+}
+
+// The first point of a batch with a clamped coordinate outside its range
+// (host coordinates), or -1.
+int64_t first_outside_host(const GridData& g, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt) {
+    for (int64_t i = 0; i < npts; ++i)
+        for (int d = 0; d < g.dim; ++d) {
+            const DimSetup& ds = g.setup[d];
+            const double x = coords[d * cs_dim + i * cs_pt];
+            if (!ds.periodic && !(x >= ds.lo && x <= ds.hi)) return i;
+        }
+    return -1;
+}
+
+// Device scratch of one evaluation call (bounds flag, grouping records
+// and tables): from the stream-ordered pool, or carved out of a fixed
+// buffer that the caller owns for the call.
+struct Scratch {
+    hipStream_t stream = nullptr;
+    unsigned char* base = nullptr;  // fixed buffer, or null: the pool
+    size_t cap = 0, used = 0;
+
+    bool fixed() const { return base != nullptr; }
+    bool fits(size_t bytes) const { return !fixed() || (used + 255) / 256 * 256 + bytes <= cap; }
+    hipError_t get(size_t bytes, void** out) {
+        if (!fixed()) return hipMallocAsync(out, bytes, stream);
+        const size_t at = (used + 255) / 256 * 256;
+        if (at + bytes > cap) return hipErrorOutOfMemory;
+        *out = base + at;
+        used = at + bytes;
+        return hipSuccess;
+    }
+    void put(void* p) {
+        if (!fixed() && p) (void)hipFreeAsync(p, stream);
+    }
+};
+
+int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt, int32_t nout,
+              const int32_t* orders, double* out, int64_t os_out, int64_t os_pt, uint32_t flags, hipStream_t stream,
+              Scratch& sc);
+
+}  // namespace
+
+extern "C" {
+
 int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt, int32_t nout,
               const int32_t* orders, double* out, int64_t os_out, int64_t os_pt, uint32_t flags, void* stream_) {
+    Scratch pool;
+    pool.stream = static_cast<hipStream_t>(stream_);
+    return eval_call(s, npts, coords, cs_dim, cs_pt, nout, orders, out, os_out, os_pt, flags, pool.stream, pool);
+}
+
+
+}  // extern "C"
+
+namespace {
+
+int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt, int32_t nout,
+              const int32_t* orders, double* out, int64_t os_out, int64_t os_pt, uint32_t flags, hipStream_t stream,
+              Scratch& sc) {
     if (!s || !orders || nout < 1) return fail(INTP_E_INVALID, "null argument or no outputs");
     if (npts < 0) return fail(INTP_E_INVALID, "negative point count");
     if (npts == 0) return INTP_OK;
@@ -779,7 +937,20 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     if (cs_dim < 0 || cs_pt < 0 || os_out < 0 || os_pt < 0) return fail(INTP_E_INVALID, "negative stride");
     if ((flags & INTP_EVAL_GROUPED) && (flags & INTP_EVAL_UNGROUPED))
         return fail(INTP_E_INVALID, "INTP_EVAL_GROUPED and INTP_EVAL_UNGROUPED together");
-    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    if (flags & INTP_EVAL_HOST_MIRROR) {
+        // on this thread, from the host copy of the table
+        if (flags & INTP_EVAL_OUT_F32) return fail(INTP_E_INVALID, "host evaluation writes FP64 results");
+        int rc = ensure_mirror(s);
+        if (rc != INTP_OK) return rc;
+        const int64_t bad = host_eval_points(host_view(*s), npts, coords, cs_dim, cs_pt, nout, orders, out, os_out,
+                                             os_pt, (flags & INTP_EVAL_BOUNDS_CHECK) != 0);
+        if (bad >= 0) {
+            double x[kMaxDim];
+            for (int d = 0; d < D; ++d) x[d] = coords[d * cs_dim + bad * cs_pt];
+            return domain_error_at(g, bad, x);
+        }
+        return INTP_OK;
+    }
     DeviceScope scope(g.device);
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
 
@@ -796,25 +967,32 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     const bool out_f32 = (flags & INTP_EVAL_OUT_F32) != 0;
     const size_t out_elem = out_f32 ? sizeof(float) : sizeof(double);
     const size_t cext = static_cast<size_t>((D - 1) * cs_dim + (npts - 1) * cs_pt + 1);
-    const size_t oext = static_cast<size_t>((nout - 1) * os_out + (npts - 1) * os_pt + 1);
+    // Host outputs are staged in a dense device array [nout][npts] (or in
+    // the caller's own layout when that is dense already) and copied back
+    // once; nothing is copied to the device first.
+    const bool dense_out = (os_pt == 1 && os_out == npts) || (os_out == 1 && os_pt == nout);
+    const size_t oext = static_cast<size_t>(nout) * static_cast<size_t>(npts);
     double* dcoords = nullptr;
     void* dout = nullptr;
     int* dbad = nullptr;
     auto cleanup = [&]() {
         if (dcoords) (void)hipFreeAsync(dcoords, stream);
         if (dout) (void)hipFreeAsync(dout, stream);
-        if (dbad) (void)hipFreeAsync(dbad, stream);
+        sc.put(dbad);
     };
     if (host) {
         hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&dcoords), cext * sizeof(double), stream);
         if (e == hipSuccess) e = hipMallocAsync(&dout, oext * out_elem, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dcoords, coords, cext * sizeof(double), hipMemcpyHostToDevice, stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(dout, out, oext * out_elem, hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) {
             cleanup();
             return hip_fail(e, "staging host buffers");
         }
         p.coords = dcoords;
+        if (!dense_out) {
+            p.os_out = npts;
+            p.os_pt = 1;
+        }
     } else {
         p.coords = coords;
     }
@@ -824,8 +1002,10 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     else
         p.out = static_cast<double*>(out_dev);
     if (flags & INTP_EVAL_BOUNDS_CHECK) {
-        hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&dbad), sizeof(int), stream);
-        if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), stream);
+        hipError_t e = sc.get(sizeof(int), reinterpret_cast<void**>(&dbad));
+        // cleared by a kernel (graph-capturable; a fixed buffer keeps the
+        // previous call's contents)
+        if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dbad), 0u, 1, stream);
         if (e != hipSuccess) {
             cleanup();
             return hip_fail(e, "bounds flag");
@@ -863,7 +1043,7 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         p.slot[0] = -1;
     }
     {
-        bool packed = !out_f32 && os_out == 1 && os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;
+        bool packed = !out_f32 && p.os_out == 1 && p.os_pt % 2 == 0 && reinterpret_cast<uintptr_t>(p.out) % 16 == 0;
         for (int i = 0; i <= D; ++i) packed = packed && p.slot[i] == i;
         p.out_packed = packed ? 1 : 0;
     }
@@ -885,14 +1065,82 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     BinRecord* rec_b = nullptr;
     uint32_t* bin_scratch = nullptr;
     auto free_bins = [&]() {
-        if (rec_a) (void)hipFreeAsync(rec_a, stream);
-        if (rec_b) (void)hipFreeAsync(rec_b, stream);
-        if (bin_scratch) (void)hipFreeAsync(bin_scratch, stream);
+        sc.put(rec_a);
+        sc.put(rec_b);
+        sc.put(bin_scratch);
     };
     const bool group = (flags & INTP_EVAL_GROUPED) ? npts > 0
                        : !(flags & INTP_EVAL_UNGROUPED) && group_pays(D, g.total * s->ctrl_bytes, npts);
-    if (fast && D >= 2 && group && npts <= 0xFFFFFFFFll &&
-        bin_scan_tiles(npts) <= 4096) {
+    BrickGrid ib{};
+    int64_t item_patch = 0;
+    int32_t item_bins = 0;
+    bool one_pass = fast && D >= 2 && group && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_GROUP_TWO_PASS) &&
+                    item_bricks(g, s->ctrl_bytes, &ib, &item_patch, &item_bins);
+    bool two_pass = !one_pass && fast && D >= 2 && group && npts <= 0xFFFFFFFFll && bin_scan_tiles(npts) <= 4096;
+    if (sc.fixed() && (one_pass || two_pass)) {
+        // the grouping's scratch must fit the fixed buffer; otherwise the
+        // points are evaluated in caller order (same results), unless the
+        // caller forced grouping
+        const size_t need = one_pass ? static_cast<size_t>(one_pass_scratch(npts, item_bins))
+                                     : static_cast<size_t>(two_pass_scratch(npts));
+        if (!sc.fits(need)) {
+            if (flags & INTP_EVAL_GROUPED) {
+                if (timing)
+                    for (hipEvent_t ev : timed.ev) t_spare_events.emplace_back(g.device, ev);
+                cleanup();
+                return fail(INTP_E_INVALID, "grouping " + std::to_string(npts) + " points needs " +
+                                                std::to_string(need) + " bytes of scratch; the static buffer holds " +
+                                                std::to_string(sc.cap - sc.used));
+            }
+            one_pass = two_pass = false;
+        }
+    }
+    if (one_pass) {
+        GroupParams gp{};
+        for (int d = 0; d < D; ++d) gp.dim[d] = p.dim[d];
+        gp.bricks = ib;
+        gp.dim_count = D;
+        gp.order = g.order;
+        gp.nbins = item_bins;
+        gp.nblocks = static_cast<int32_t>(group_blocks(npts, item_bins));
+        gp.coords = p.coords;
+        gp.cs_dim = p.cs_dim;
+        gp.cs_pt = p.cs_pt;
+        gp.npts = npts;
+        gp.max_items = group_max_items(npts, item_bins);
+        const int64_t nh = static_cast<int64_t>(item_bins) * gp.nblocks;
+        const int64_t nt = scan_tile_sums(nh);
+        auto up = [](int64_t bytes) { return (bytes + 255) / 256 * 256; };
+        const int64_t b_rec = up(npts * static_cast<int64_t>(sizeof(BinRecord)));
+        const int64_t b_hist = up(nh * 4), b_tiles = up(nt * 4);
+        const int64_t b_items = up(gp.max_items * static_cast<int64_t>(sizeof(BrickItem)));
+        e = sc.get(static_cast<size_t>(one_pass_scratch(npts, item_bins)), reinterpret_cast<void**>(&rec_a));
+        if (e == hipSuccess) {
+            unsigned char* base = reinterpret_cast<unsigned char*>(rec_a);
+            gp.rec = rec_a;
+            gp.hist = reinterpret_cast<uint32_t*>(base + b_rec);
+            gp.offs = reinterpret_cast<uint32_t*>(base + b_rec + b_hist);
+            gp.tile_sums = reinterpret_cast<uint32_t*>(base + b_rec + 2 * b_hist);
+            gp.items = reinterpret_cast<BrickItem*>(base + b_rec + 2 * b_hist + b_tiles);
+            gp.item_count = reinterpret_cast<uint32_t*>(base + b_rec + 2 * b_hist + b_tiles + b_items);
+            e = launch_grouping(gp, stream);
+        }
+        if (e != hipSuccess) {
+            free_bins();
+            cleanup();
+            if (timing)
+                for (hipEvent_t ev : timed.ev) t_spare_events.emplace_back(g.device, ev);
+            return hip_fail(e, "grouping query points");
+        }
+        p.coords = reinterpret_cast<const double*>(rec_a);
+        p.cs_dim = 1;
+        p.cs_pt = sizeof(BinRecord) / sizeof(double);
+        p.bricks = ib;
+        p.patch_volume = static_cast<int32_t>(item_patch);
+        p.items = gp.items;
+        p.item_count = gp.item_count;
+        p.max_items = gp.max_items;
+    } else if (two_pass) {
         int64_t patch_volume = 0;
         BinParams bp = bin_params(g, s->ctrl_bytes, &patch_volume);
         bp.coords = p.coords;
@@ -900,13 +1148,10 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         bp.cs_pt = p.cs_pt;
         bp.npts = npts;
         const int64_t nh = bin_hist_entries(npts), nt = bin_scan_tiles(npts);
-        e = hipMallocAsync(reinterpret_cast<void**>(&rec_a), static_cast<size_t>(npts) * sizeof(BinRecord), stream);
+        e = sc.get(static_cast<size_t>(npts) * sizeof(BinRecord), reinterpret_cast<void**>(&rec_a));
+        if (e == hipSuccess) e = sc.get(static_cast<size_t>(npts) * sizeof(BinRecord), reinterpret_cast<void**>(&rec_b));
         if (e == hipSuccess)
-            e = hipMallocAsync(reinterpret_cast<void**>(&rec_b), static_cast<size_t>(npts) * sizeof(BinRecord),
-                               stream);
-        if (e == hipSuccess)
-            e = hipMallocAsync(reinterpret_cast<void**>(&bin_scratch), static_cast<size_t>(2 * nh + nt + npts) * 4,
-                               stream);
+            e = sc.get(static_cast<size_t>(2 * nh + nt + npts) * 4, reinterpret_cast<void**>(&bin_scratch));
         if (e == hipSuccess)
             e = launch_binning(bp, bin_scratch, bin_scratch + nh, bin_scratch + 2 * nh, bin_scratch + 2 * nh + nt,
                                rec_a, rec_b, stream);
@@ -962,8 +1207,14 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
             return hip_fail(e, "bounds flag copy");
         }
     }
+    std::vector<unsigned char> staged;
     if (host) {
-        e = hipMemcpyAsync(out, dout, oext * out_elem, hipMemcpyDeviceToHost, stream);
+        void* dst = out;
+        if (!dense_out) {
+            staged.resize(oext * out_elem);
+            dst = staged.data();
+        }
+        e = hipMemcpyAsync(dst, dout, oext * out_elem, hipMemcpyDeviceToHost, stream);
         if (e != hipSuccess) {
             cleanup();
             return hip_fail(e, "output copy");
@@ -971,7 +1222,211 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     }
     cleanup();
     if (host || dbad) HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (host && !dense_out) {
+        // [nout][npts] -> the caller's strides
+        for (int o = 0; o < nout; ++o)
+            for (int64_t i = 0; i < npts; ++i) {
+                const size_t src = static_cast<size_t>(o) * static_cast<size_t>(npts) + static_cast<size_t>(i);
+                if (out_f32)
+                    reinterpret_cast<float*>(out)[o * os_out + i * os_pt] = reinterpret_cast<const float*>(staged.data())[src];
+                else
+                    out[o * os_out + i * os_pt] = reinterpret_cast<const double*>(staged.data())[src];
+            }
+    }
+    if (bad) {
+        // name the first offending point
+        int64_t first = -1;
+        if (host) {
+            first = first_outside_host(g, npts, coords, cs_dim, cs_pt);
+        } else {
+            uint64_t* dfirst = nullptr;
+            uint64_t hfirst = ~uint64_t{0};
+            BoundsParams bp{};
+            for (int d = 0; d < D; ++d) bp.dim[d] = p.dim[d];
+            bp.ndim = D;
+            bp.coords = coords;
+            bp.cs_dim = cs_dim;
+            bp.cs_pt = cs_pt;
+            bp.npts = npts;
+            sc.used = 0;  // the call's launches are done (synchronized above)
+            e = sc.get(sizeof(uint64_t), reinterpret_cast<void**>(&dfirst));
+            if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dfirst), ~0u, 2, stream);
+            bp.first = dfirst;
+            if (e == hipSuccess) e = launch_first_outside(bp, stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(&hfirst, dfirst, sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+            sc.put(dfirst);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e == hipSuccess && hfirst != ~uint64_t{0}) first = static_cast<int64_t>(hfirst);
+            if (first >= 0) {
+                double x[kMaxDim] = {};
+                for (int d = 0; d < D && e == hipSuccess; ++d)
+                    e = hipMemcpy(&x[d], coords + d * cs_dim + first * cs_pt, sizeof(double), hipMemcpyDeviceToHost);
+                if (e == hipSuccess) return domain_error_at(g, first, x);
+            }
+        }
+        if (host && first >= 0) {
+            double x[kMaxDim];
+            for (int d = 0; d < D; ++d) x[d] = coords[d * cs_dim + first * cs_pt];
+            return domain_error_at(g, first, x);
+        }
+        return fail(INTP_E_DOMAIN, "Given coordinate out of interpolation function range!");
+    }
+    return INTP_OK;
+}
+
+}  // namespace
+
+struct intp_proxy_s {
+    std::shared_ptr<GridData> grid;
+    int64_t npts = 0;
+    bool host = false;
+    bool uniform = false;  // device weights from the uniform-grid path
+    std::vector<int32_t> first_h;
+    std::vector<double> weights_h;
+    unsigned char* dev = nullptr;  // first (int32) then weights (double)
+    int32_t* first_d = nullptr;
+    double* weights_d = nullptr;
+    UseTracker uses;
+    ~intp_proxy_s() {
+        if (dev) release_after(grid->device, dev, uses);
+    }
+};
+
+namespace {
+
+// Splines on equal grids: the same tables, or equal dimension, order,
+// sizes, periodicity and knot vectors.
+bool same_grid(const GridData& a, const GridData& b) {
+    if (&a == &b) return true;
+    if (a.dim != b.dim || a.order != b.order) return false;
+    for (int d = 0; d < a.dim; ++d) {
+        const DimSetup &x = a.setup[d], &y = b.setup[d];
+        if (x.n != y.n || x.periodic != y.periodic || x.uniform != y.uniform || x.knots.size() != y.knots.size() ||
+            std::memcmp(x.knots.data(), y.knots.data(), x.knots.size() * sizeof(double)) != 0)
+            return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int intp_proxy_create(intp_template t, intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
+                      int64_t cs_pt, uint32_t flags, intp_proxy* out, void* stream_) {
+    if (!out || (!t == !s)) return fail(INTP_E_INVALID, "null output, or not exactly one of template and spline");
+    if (npts < 0 || (npts > 0 && !coords)) return fail(INTP_E_INVALID, "null coordinates or negative point count");
+    if (cs_dim < 0 || cs_pt < 0) return fail(INTP_E_INVALID, "negative stride");
+    if (npts >= (int64_t{1} << 31)) return fail(INTP_E_UNSUPPORTED, "a proxy holds fewer than 2^31 points");
+    const std::shared_ptr<GridData>& gp = t ? t->grid : s->grid;
+    const GridData& g = *gp;
+    const int D = g.dim, w = g.order + 1;
+    std::unique_ptr<intp_proxy_s> p(new intp_proxy_s());
+    p->grid = gp;
+    p->npts = npts;
+    p->host = (flags & INTP_EVAL_HOST_MEM) != 0;
+    const bool bounds = (flags & INTP_EVAL_BOUNDS_CHECK) != 0;
+    if (p->host) {
+        p->first_h.resize(static_cast<size_t>(npts * D));
+        p->weights_h.resize(static_cast<size_t>(npts * D * w));
+        HostSpline hs{};
+        hs.dim = D;
+        hs.order = g.order;
+        for (int d = 0; d < D; ++d) hs.dims[d] = &g.setup[d];
+        const int64_t bad =
+            host_proxy_weights(hs, npts, coords, cs_dim, cs_pt, bounds, p->first_h.data(), p->weights_h.data());
+        if (bad >= 0) {
+            double x[kMaxDim];
+            for (int d = 0; d < D; ++d) x[d] = coords[d * cs_dim + bad * cs_pt];
+            return domain_error_at(g, bad, x);
+        }
+        *out = p.release();
+        return INTP_OK;
+    }
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    DeviceScope scope(g.device);
+    if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
+    const size_t first_bytes = (static_cast<size_t>(npts * D) * sizeof(int32_t) + 255) / 256 * 256;
+    const size_t bytes = first_bytes + static_cast<size_t>(npts * D * w) * sizeof(double) + 256;
+    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&p->dev), bytes, stream), "hipMallocAsync (proxy)");
+    p->first_d = reinterpret_cast<int32_t*>(p->dev);
+    p->weights_d = reinterpret_cast<double*>(p->dev + first_bytes);
+    p->uniform = uniform_path(g);
+    ProxyParams pp{};
+    for (int d = 0; d < D; ++d) pp.dim[d] = dim_params(g, d);
+    pp.npts = npts;
+    pp.coords = coords;
+    pp.cs_dim = cs_dim;
+    pp.cs_pt = cs_pt;
+    pp.first = p->first_d;
+    pp.weights = p->weights_d;
+    int* dbad = nullptr;
+    hipError_t e = hipSuccess;
+    if (bounds) {
+        e = hipMallocAsync(reinterpret_cast<void**>(&dbad), sizeof(int), stream);
+        if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), stream);
+        pp.bad = dbad;
+    }
+    if (e == hipSuccess) e = launch_proxy_create(D, g.order, p->uniform, pp, stream);
+    if (e == hipSuccess) e = p->uses.record(stream);
+    if (e == hipSuccess) e = gp->uses.record(stream);
+    int bad = 0;
+    if (dbad) {
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, stream);
+        (void)hipFreeAsync(dbad, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    }
+    if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "no proxy kernel for this dimension/order");
+    HIP_TRY(e, "proxy launch");
     if (bad) return fail(INTP_E_DOMAIN, "Given coordinate out of interpolation function range!");
+    *out = p.release();
+    return INTP_OK;
+}
+
+int intp_proxy_eval(intp_proxy p, intp_spline s, double* out, int64_t os_pt, uint32_t flags, void* stream_) {
+    if (!p || !s) return fail(INTP_E_INVALID, "null proxy or spline");
+    if (p->npts == 0) return INTP_OK;
+    if (!out || os_pt < 0) return fail(INTP_E_INVALID, "null output or negative stride");
+    if (!same_grid(*p->grid, *s->grid)) return fail(INTP_E_INVALID, "the spline is not on the proxy's grid");
+    if (p->host) {
+        if (flags & INTP_EVAL_OUT_F32) return fail(INTP_E_INVALID, "host proxies write FP64 results");
+        int rc = ensure_mirror(s);
+        if (rc != INTP_OK) return rc;
+        host_proxy_apply(host_view(*s), p->npts, p->first_h.data(), p->weights_h.data(), out, os_pt);
+        return INTP_OK;
+    }
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    const GridData& g = *s->grid;
+    DeviceScope scope(g.device);
+    if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
+    ProxyParams pp{};
+    for (int d = 0; d < g.dim; ++d) pp.dim[d] = dim_params(g, d);
+    pp.npts = p->npts;
+    pp.first = p->first_d;
+    pp.weights = p->weights_d;
+    pp.ctrl = s->ctrl;
+    if (flags & INTP_EVAL_OUT_F32)
+        pp.out32 = reinterpret_cast<float*>(out);
+    else
+        pp.out = out;
+    pp.os_pt = os_pt;
+    p->uses.order_before(stream);
+    hipError_t e = launch_proxy_apply(g.dim, g.order, p->uniform, s->ctrl_bytes == 4, pp, stream);
+    if (e == hipErrorNotSupported) return fail(INTP_E_UNSUPPORTED, "no proxy kernel for this dimension/order");
+    HIP_TRY(e, "proxy launch");
+    HIP_TRY(p->uses.record(stream), "hipEventRecord");
+    HIP_TRY(s->uses.record(stream), "hipEventRecord");
+    return INTP_OK;
+}
+
+int intp_proxy_points(intp_proxy p, int64_t* npts) {
+    if (!p || !npts) return fail(INTP_E_INVALID, "null argument");
+    *npts = p->npts;
+    return INTP_OK;
+}
+
+int intp_proxy_destroy(intp_proxy p) {
+    delete p;
     return INTP_OK;
 }
 

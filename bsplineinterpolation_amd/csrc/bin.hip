@@ -29,30 +29,8 @@ constexpr int kChunk = kThreads * kPerThread;  // points per block
 constexpr int kRadix = 256;
 constexpr int kScanTile = 4096;  // entries per scan block
 
-// Brick of a point: per dimension, the first control point of its
-// stencil shifted down to the brick.  The span comes from the reference's
-// O(1) hint (Interpolation.hpp:277-295) without the exact-knot correction
-// of the evaluation: a point keyed one span off at a knot only leaves its
-// brick's patch (the evaluation then reads it from HBM); results never
-// depend on the key.
 __device__ __forceinline__ uint32_t brick_key(const BinParams& p, const double* x) {
-    uint32_t key = 0;
-#pragma unroll
-    for (int d = 0; d < kMaxDim; ++d) {
-        if (d >= p.dim_count) break;
-        const DimParams& dp = p.dim[d];
-        double xd = x[d];
-        wrap_or_check(dp, xd, nullptr);
-        const int last = dp.nt - p.order - 2;
-        double v = (xd - dp.lo) * dp.inv_dx - dp.shift;
-        v = v > 0.0 ? v : 0.0;  // NaN -> 0
-        v = v < static_cast<double>(last) ? v : static_cast<double>(last);
-        int start = min(last, static_cast<int>(ceil(v)) + p.order) - p.order;
-        if (start >= dp.n) start -= static_cast<int>(dp.n);  // periodic, even order: the last span wraps
-        key = key * static_cast<uint32_t>(p.bricks.nbrick[d]) +
-              (static_cast<uint32_t>(start) >> p.bricks.brick_shift[d]);
-    }
-    return key;
+    return brick_key_of(p.dim, p.dim_count, p.order, p.bricks, x);
 }
 
 __device__ __forceinline__ void load_point(const BinParams& p, int64_t i, double* x) {
@@ -298,6 +276,12 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t* tile_sums
 }
 
 }  // namespace
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* tile_sums, int64_t n, hipStream_t s) {
+    return exclusive_scan(in, out, tile_sums, n, s);
+}
+
+int64_t scan_tile_sums(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 int64_t bin_chunk_points() { return kChunk; }
 

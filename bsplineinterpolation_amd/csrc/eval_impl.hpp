@@ -676,6 +676,106 @@ __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
                                           load_record(reinterpret_cast<const BinRecord*>(p.coords) + pt));
 }
 
+// One workgroup per brick item (one-pass grouping): the item's records
+// all belong to one brick, whose (B_d + K) patch of control points is
+// staged once in dynamic LDS (wrapped on periodic dimensions, zero past
+// the end of clamped ones, where no stencil of the brick reaches); each
+// thread's kBrickPointsPerThread records are loaded before the staging so
+// that both sets of loads are in flight together.
+// Waves per SIMD the brick kernel is built for: 4 (at most 128 VGPRs)
+// up to order 3, else 2.
+template <int D, int K>
+constexpr int brick_waves() {
+    return K <= 3 ? 4 : 2;
+}
+
+template <int D, int K, bool GRAD, bool LASTC, typename T>
+__global__ __launch_bounds__(kBrickThreads, (brick_waves<D, K>())) void eval_brick_kernel(EvalParams p) {
+    extern __shared__ __align__(16) unsigned char brick_smem[];
+    T* s_patch = reinterpret_cast<T*>(brick_smem);
+    const int64_t item_id = blockIdx.x;
+    if (item_id >= static_cast<int64_t>(*p.item_count)) return;
+    const BrickItem it = p.items[item_id];
+    const BinRecord* __restrict__ recs = reinterpret_cast<const BinRecord*>(p.coords);
+    // the thread's first record is in flight during the staging; each
+    // later one during the previous point's evaluation
+    BinRecord next{};
+    if (threadIdx.x < it.count) next = load_record(recs + it.start + threadIdx.x);
+    // brick of the item
+    int bstart[D], bsize[D], psize[D];
+    uint32_t lstride[D];
+    uint32_t rem = it.bin;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        const uint32_t nb = static_cast<uint32_t>(p.bricks.nbrick[d]);
+        const uint32_t b = rem % nb;
+        rem /= nb;
+        bstart[d] = static_cast<int>(b << p.bricks.brick_shift[d]);
+        bsize[d] = 1 << p.bricks.brick_shift[d];
+        psize[d] = bsize[d] + K;
+    }
+    uint32_t vol = 1;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        lstride[d] = vol;
+        vol *= static_cast<uint32_t>(psize[d]);
+    }
+    const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
+    if (vol <= static_cast<uint32_t>(p.patch_volume) && psize[D - 1] <= kBrickThreads) {
+        // rows along the last dimension: thread t owns position t % P of a
+        // row and walks rows t / P, t / P + R, ... (R = threads / P)
+        const int P = psize[D - 1];
+        const int R = kBrickThreads / P;
+        const int i_last = threadIdx.x % P;
+        int row = threadIdx.x / P;
+        const int nrows = static_cast<int>(vol) / P;
+        int64_t gl = bstart[D - 1] + i_last;
+        const int64_t n_last = p.dim[D - 1].n;
+        bool ok_last = row < R;
+        if (gl >= n_last) {
+            if (p.dim[D - 1].periodic)
+                while (gl >= n_last) gl -= n_last;
+            else
+                ok_last = false;
+        }
+        if (ok_last) {
+            for (; row < nrows; row += R) {
+                // row -> outer indices (C order over dimensions 0..D-2)
+                int r = row;
+                int64_t goff = gl;
+                bool ok = true;
+#pragma unroll
+                for (int d = D - 2; d >= 0; --d) {
+                    const int i = r % psize[d];
+                    r /= psize[d];
+                    int64_t g = bstart[d] + i;
+                    if (g >= p.dim[d].n) {
+                        if (p.dim[d].periodic)
+                            while (g >= p.dim[d].n) g -= p.dim[d].n;
+                        else
+                            ok = false;
+                    }
+                    goff += g * p.dim[d].stride;
+                }
+                if (ok) s_patch[static_cast<uint32_t>(row) * static_cast<uint32_t>(P) + i_last] = ctrl[goff];
+            }
+        }
+    } else {
+        // no room: every point reads HBM
+#pragma unroll
+        for (int d = 0; d < D; ++d) bsize[d] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBrickPointsPerThread; ++j) {
+        const uint32_t q = j * kBrickThreads + threadIdx.x;
+        const BinRecord cur = next;
+        if (j + 1 < kBrickPointsPerThread && q + kBrickThreads < it.count)
+            next = load_record(recs + it.start + q + kBrickThreads);
+        if (q < it.count) patch_point<D, K, GRAD, LASTC, T>(p, s_patch, bstart, bsize, lstride, cur);
+    }
+}
+
 template <void (*KF)(EvalParams)>
 hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
     const int64_t blocks = (p.npts + kEvalBlock - 1) / kEvalBlock;
@@ -685,8 +785,27 @@ hipError_t launch_chunks(const EvalParams& p, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <void (*KF)(EvalParams)>
+hipError_t launch_items(const EvalParams& p, size_t lds, hipStream_t s) {
+    if (p.max_items <= 0) return hipSuccess;
+    if (p.max_items > 0x7FFFFFFF || lds > static_cast<size_t>(kBrickPatchBytes)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(KF, dim3(static_cast<unsigned>(p.max_items)), dim3(kBrickThreads), lds, s, p);
+    return hipGetLastError();
+}
+
 template <int D, int K, bool LASTC, typename T>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    if (uniform && p.items) {
+        // brick items of the one-pass grouping (2D and 3D grids)
+        if constexpr (D >= 2) {
+            if (!p.item_count || p.cs_dim != 1 || p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
+                return hipErrorInvalidValue;
+            const size_t lds = static_cast<size_t>(p.patch_volume) * sizeof(T);
+            if (grad) return launch_items<eval_brick_kernel<D, K, true, LASTC, T>>(p, lds, s);
+            return launch_items<eval_brick_kernel<D, K, false, LASTC, T>>(p, lds, s);
+        }
+        return hipErrorInvalidValue;
+    }
     if (uniform && p.patch_volume > 0) {
         // the patch kernels read the grouping passes' records
         if (D < 2 || p.patch_volume > patch_max<T>() || !p.perm || !p.key || p.cs_dim != 1 ||

@@ -44,6 +44,12 @@ struct BrickGrid {
     int32_t nbrick[kMaxDim];
 };
 
+// A run of grouped points that share one brick (one-pass grouping,
+// group.hip): records start .. start + count - 1, brick `bin`.
+struct BrickItem {
+    uint32_t start, count, bin, pad_;
+};
+
 struct EvalParams {
     DimParams dim[kMaxDim];
     const void* ctrl;  // control points: double, or float for a refined build
@@ -75,6 +81,12 @@ struct EvalParams {
     // 16-byte aligned (os_out == 1, slot[s] == s, os_pt even): written as
     // 16-byte stores
     int32_t out_packed;
+    // brick-item evaluation (one-pass grouping): items[i] for i <
+    // *item_count (at most max_items launched workgroups), each staging
+    // its brick's patch_volume elements in dynamic LDS
+    const BrickItem* items;
+    const uint32_t* item_count;
+    int64_t max_items;
 };
 
 // A query point carried through the grouping passes (32 bytes).
@@ -98,6 +110,41 @@ struct BinParams {
     const uint32_t* key_in;  // pass 1: keys of rec_in, dense
     uint32_t* key_out;       // pass 0: keys of rec_out, dense
 };
+
+// Evaluation proxies (proxy.hip): create fills first/weights for npts
+// points; apply contracts them with a spline's table into out / out32.
+//   first[i * dim + d]                      = span - order (before wrap)
+//   weights[(i * dim + d) * (order + 1) + m] = value basis
+struct ProxyParams {
+    DimParams dim[kMaxDim];
+    int64_t npts;
+    const double* coords;
+    int64_t cs_dim, cs_pt;
+    int32_t* first;
+    double* weights;
+    int* bad;
+    const void* ctrl;
+    double* out;
+    float* out32;
+    int64_t os_pt;
+};
+hipError_t launch_proxy_create(int dim, int order, bool uniform, const ProxyParams& p, hipStream_t s);
+
+// The smallest index of a point with a clamped coordinate outside its
+// range (the bounds check's offending point), atomically minimized into
+// *first (initialized to all ones by the caller).
+struct BoundsParams {
+    DimParams dim[kMaxDim];
+    int32_t ndim;
+    const double* coords;
+    int64_t cs_dim, cs_pt;
+    int64_t npts;
+    uint64_t* first;
+};
+hipError_t launch_first_outside(const BoundsParams& p, hipStream_t s);
+// p[0 .. n-1] = value (a kernel, so that it can be captured in a graph).
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, int64_t n, hipStream_t s);
+hipError_t launch_proxy_apply(int dim, int order, bool uniform, bool f32, const ProxyParams& p, hipStream_t s);
 
 // Line solves along one axis of a C-order table viewed as
 // (outer, n, inner): line (o, i) holds elements o*n*inner + j*inner + i.
@@ -170,6 +217,43 @@ hipError_t launch_solve(int bw, bool cyclic, SolveTypes types, const SolveParams
 // current device (a power of two up to 64) for compute elements of
 // `compute_bytes`, 0 if a line does not fit.
 int solve_tile_lines(int64_t n, int compute_bytes);
+
+// One-pass grouping (group.hip): per-block LDS histograms of the brick
+// keys over fixed ranges of points, one exclusive scan of the bin-major
+// counts, a scatter of 32-byte records to their bins with LDS-atomic
+// cursors, and the list of brick items (runs of at most kBrickItemPoints
+// records of one brick) for the evaluation kernel.
+constexpr int kGroupThreads = 512;
+constexpr int kGroupMaxBins = 16384;      // LDS counters of the hist/scatter blocks
+constexpr int kBrickThreads = 512;
+constexpr int kBrickPointsPerThread = 4;
+constexpr int kBrickItemPoints = kBrickThreads * kBrickPointsPerThread;
+constexpr int kBrickPatchBytes = 64 * 1024 - 256;  // dynamic LDS of the brick kernel
+
+struct GroupParams {
+    DimParams dim[kMaxDim];
+    BrickGrid bricks;
+    int32_t dim_count;
+    int32_t order;
+    int32_t nbins;
+    int32_t nblocks;    // blocks of the histogram and scatter passes
+    int64_t per_block;  // points per block
+    const double* coords;
+    int64_t cs_dim, cs_pt;
+    int64_t npts;
+    uint32_t* hist;       // nbins * nblocks counts, bin-major
+    uint32_t* offs;       // their exclusive scan
+    uint32_t* tile_sums;  // scan scratch
+    BinRecord* rec;       // npts records, grouped by bin
+    BrickItem* items;     // at most max_items
+    uint32_t* item_count;
+    int64_t max_items;
+};
+int64_t group_blocks(int64_t npts, int nbins);
+int64_t group_max_items(int64_t npts, int nbins);
+hipError_t launch_grouping(GroupParams p, hipStream_t s);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* tile_sums, int64_t n, hipStream_t s);
+int64_t scan_tile_sums(int64_t n);
 
 // Grouping of query points by brick (bin.hip): scratch sizes and launch.
 int64_t bin_chunk_points();

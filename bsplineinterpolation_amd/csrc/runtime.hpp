@@ -91,6 +91,11 @@ inline DeviceRuntime* device_runtime(int device) {
                 keep = kPoolKeepBytes;
                 (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
             }
+            // A block freed on one stream is handed to an allocation on
+            // another stream only once the free has completed, never by
+            // dependencies the pool inserts between the two streams.
+            int no = 0;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no);
         }
         if (!r.release && hipStreamCreateWithFlags(&r.release, hipStreamNonBlocking) != hipSuccess) return nullptr;
         if (!r.setup && hipStreamCreateWithFlags(&r.setup, hipStreamNonBlocking) != hipSuccess) return nullptr;

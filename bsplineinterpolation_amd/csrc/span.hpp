@@ -60,4 +60,34 @@ __device__ __forceinline__ int uniform_span(const DimParams& dp, int k, double x
     return mu;
 }
 
+// Brick of a point: per dimension, the first control point of its
+// stencil shifted down to the brick, combined in C order.  The span comes
+// from the reference's O(1) hint (Interpolation.hpp:277-295) without the
+// exact-knot correction of the evaluation: a point keyed one span off at a
+// knot only leaves its brick's patch (the evaluation then reads it from
+// HBM); results never depend on the key.
+// The counting and scatter passes compute every key twice and rely on
+// both giving the same value: no FMA contraction, whatever the caller's
+// compilation.
+__device__ __forceinline__ uint32_t brick_key_of(const DimParams* dims, int ndim, int order, const BrickGrid& bricks,
+                                                 const double* x) {
+#pragma clang fp contract(off)
+    uint32_t key = 0;
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) {
+        if (d >= ndim) break;
+        const DimParams& dp = dims[d];
+        double xd = x[d];
+        wrap_or_check(dp, xd, nullptr);
+        const int last = dp.nt - order - 2;
+        double v = (xd - dp.lo) * dp.inv_dx - dp.shift;
+        v = v > 0.0 ? v : 0.0;  // NaN -> 0
+        v = v < static_cast<double>(last) ? v : static_cast<double>(last);
+        int start = min(last, static_cast<int>(ceil(v)) + order) - order;
+        if (start >= dp.n) start -= static_cast<int>(dp.n);  // periodic, even order: the last span wraps
+        key = key * static_cast<uint32_t>(bricks.nbrick[d]) + (static_cast<uint32_t>(start) >> bricks.brick_shift[d]);
+    }
+    return key;
+}
+
 }  // namespace intp_amd

@@ -13,7 +13,8 @@
 //    Interpolation.hpp:49-125;
 //  * operator(), at, derivative and derivative_at in all three argument
 //    forms, Interpolation.hpp:132-244; periodicity / uniform / range /
-//    get_order, :248-267; eval_proxy, :493-506;
+//    spline / get_order, :248-267; eval_proxy, :493-506;
+//  * intp::BSpline<T, D, O, U> as a read-only view (BSpline.hpp:32-760);
 //  * intp::InterpolationFunctionTemplate<T, D, O, U> with interpolate() and
 //    eval_proxy (InterpolationTemplate.hpp:32-176), the 1D classes
 //    InterpolationFunction1D (Interpolation.hpp:509-540) and
@@ -26,11 +27,16 @@
 //    invalid arguments and HIP errors.
 // Conventions: INTP_PERIODIC_NO_DUMMY_POINT (a periodic dimension stores n
 // values, spacing (max - min) / n) and INTP_CELL_LAYOUT semantics.
-// T = U = double (the FP64 path).
+// Values: double (the FP64 path), float (the refined-FP32 form), or a
+// fixed-size vector of either (component splines on one grid);
+// coordinates double or float.
 //
-// Added: batched evaluation of many points in one call (eval(), host or
-// device buffers), which is how the GPU is meant to be used; the scalar
-// members each make one synchronous device round trip.
+// Scalar members (operator(), at, derivative, derivative_at, eval_proxy)
+// evaluate on the calling thread from a host copy of the GPU-built
+// control points, made on first use, like the reference's per-point
+// calls.  Added: batched evaluation of many points in one call (eval(),
+// host or device buffers) on the GPU, which is how the GPU is meant to be
+// used.
 //
 // Device selection: intp::set_device(d) before the first construction
 // (default 0, or the INTP_DEVICE environment variable).
@@ -253,13 +259,183 @@ class Mesh {
 
 template <typename T, std::size_t D, std::size_t O, typename U = double>
 class InterpolationFunctionTemplate;
+template <typename T, std::size_t D, std::size_t O, typename U = double>
+class InterpolationFunction;
+
+namespace detail {
+
+template <typename...>
+using void_t = void;
+
+// Value types: an arithmetic type (one component), or a fixed-size vector
+// type with a static `dim`, a `value_type` and operator[] (the reference's
+// own Vec<D, T>, test/src/include/Vec.hpp), interpolated component by
+// component on one grid.  Components are float or double.
+template <typename T, typename = void>
+struct value_traits {
+    static_assert(std::is_arithmetic<T>::value, "value type must be arithmetic or a fixed-size vector type");
+    using scalar = T;
+    static constexpr std::size_t components = 1;
+    static scalar get(const T& v, std::size_t) { return v; }
+    static void set(T& v, std::size_t, scalar x) { v = x; }
+};
+
+template <typename T>
+struct value_traits<T, void_t<decltype(T::dim), typename T::value_type>> {
+    using scalar = typename T::value_type;
+    static constexpr std::size_t components = T::dim;
+    static scalar get(const T& v, std::size_t c) { return v[c]; }
+    static void set(T& v, std::size_t c, scalar x) { v[c] = x; }
+};
+
+// An evaluation proxy's device-library handle.
+using ProxyHandle = std::shared_ptr<intp_proxy_s>;
+
+inline ProxyHandle own(intp_proxy p) {
+    return ProxyHandle(p, [](intp_proxy q) { intp_proxy_destroy(q); });
+}
+
+}  // namespace detail
+
+// ---- BSpline -----------------------------------------------------------
+//
+// Read-only view of the spline inside an InterpolationFunction (the
+// reference's BSpline<T, D, O, U>, BSpline.hpp:32-760, as returned by
+// InterpolationFunction::spline(), Interpolation.hpp:264): knot vectors,
+// control points, range, periodicity and order, and evaluation.  Knots and
+// control points are copied from the device on first access.
+// control_points() is the plain C-order table (the reference's
+// INTP_CELL_LAYOUT build returns its redundant cell-layout copy instead).
+template <typename T, std::size_t D, std::size_t O, typename U = double>
+class BSpline {
+    using traits = detail::value_traits<T>;
+    static constexpr std::size_t C = traits::components;
+
+   public:
+    using val_type = T;
+    using knot_type = U;
+    using size_type = std::size_t;
+    using diff_type = std::ptrdiff_t;
+    constexpr static size_type dim = D;
+    constexpr static size_type order = O;
+    template <typename V>
+    using DimArray = std::array<V, D>;
+    using KnotContainer = std::vector<knot_type>;
+    using knot_const_iterator = typename KnotContainer::const_iterator;
+    using control_point_type = Mesh<val_type, D>;
+
+    BSpline() = default;
+
+    inline knot_const_iterator knots_begin(size_type d) const { return host().knots[d].cbegin(); }
+    inline knot_const_iterator knots_end(size_type d) const { return host().knots[d].cend(); }
+    inline size_type knots_num(size_type d) const { return host().knots[d].size(); }
+    inline const control_point_type& control_points() const { return host().ctrl; }
+    inline const std::pair<knot_type, knot_type>& range(size_type d) const { return ranges_[d]; }
+    inline bool periodicity(size_type d) const { return info_.periodic[d] != 0; }
+    inline constexpr size_type get_order() const { return order; }
+
+    // value at coordinates (BSpline.hpp:305-382)
+    val_type operator()(DimArray<double> coords) const { return eval(coords, DimArray<size_type>{}, false); }
+    template <typename... Coords, typename = typename std::enable_if<sizeof...(Coords) == D &&
+                                                                     std::is_arithmetic<typename std::common_type<
+                                                                         Coords...>::type>::value>::type>
+    val_type operator()(Coords... x) const {
+        return eval(DimArray<double>{static_cast<double>(x)...}, DimArray<size_type>{}, false);
+    }
+    // mixed partial derivative, (coordinate, order) per dimension
+    // (BSpline.hpp:393-550)
+    val_type derivative_at(DimArray<std::pair<knot_type, size_type>> coord_deriv) const {
+        DimArray<double> x;
+        DimArray<size_type> r;
+        for (size_type d = 0; d < D; ++d) {
+            x[d] = static_cast<double>(coord_deriv[d].first);
+            r[d] = coord_deriv[d].second;
+        }
+        return eval(x, r, false);
+    }
+
+    // ---- used by InterpolationFunction ---------------------------------
+    void bind(const std::array<detail::SplineHandle, C>& splines, const intp_spline_info& info) {
+        splines_ = splines;
+        info_ = info;
+        for (size_type d = 0; d < D; ++d)
+            ranges_[d] = std::make_pair(static_cast<knot_type>(info.range_lo[d]), static_cast<knot_type>(info.range_hi[d]));
+        host_ = std::make_shared<HostCopy>();
+    }
+    // One point, derivative orders r, on the host copy of the table
+    // (INTP_EVAL_HOST_MIRROR); `bounds`: the at() check.
+    val_type eval(const DimArray<double>& x, const DimArray<size_type>& r, bool bounds) const {
+        std::array<std::int32_t, D> ri{};
+        for (size_type d = 0; d < D; ++d) ri[d] = static_cast<std::int32_t>(r[d]);
+        val_type v{};
+        for (std::size_t c = 0; c < C; ++c) {
+            double out = 0;
+            detail::check(intp_eval(handle(c), 1, x.data(), 1, static_cast<std::int64_t>(D), 1, ri.data(), &out, 1, 1,
+                                    INTP_EVAL_HOST_MIRROR | (bounds ? INTP_EVAL_BOUNDS_CHECK : 0u), nullptr));
+            traits::set(v, c, static_cast<typename traits::scalar>(out));
+        }
+        return v;
+    }
+    intp_spline handle(std::size_t c = 0) const {
+        if (!splines_[c]) throw std::runtime_error("empty InterpolationFunction");
+        return splines_[c].get();
+    }
+    const intp_spline_info& info() const {
+        (void)handle();
+        return info_;
+    }
+
+   private:
+    struct HostCopy {
+        std::once_flag once;
+        std::array<KnotContainer, D> knots;
+        control_point_type ctrl{MeshDimension<D>()};
+    };
+    std::array<detail::SplineHandle, C> splines_{};
+    intp_spline_info info_{};
+    DimArray<std::pair<knot_type, knot_type>> ranges_{};
+    std::shared_ptr<HostCopy> host_;
+
+    const HostCopy& host() const {
+        (void)handle();
+        std::call_once(host_->once, [this]() {
+            for (size_type d = 0; d < D; ++d) {
+                std::vector<double> k(static_cast<std::size_t>(info_.nknots[d]));
+                detail::check(intp_spline_download_knots(handle(), static_cast<std::int32_t>(d), k.data(),
+                                                         static_cast<std::int64_t>(k.size())));
+                host_->knots[d].assign(k.begin(), k.end());
+            }
+            typename MeshDimension<D>::index_type sizes{};
+            std::int64_t count = 1;
+            for (size_type d = 0; d < D; ++d) {
+                sizes[d] = static_cast<std::size_t>(info_.n[d]);
+                count *= info_.n[d];
+            }
+            control_point_type mesh{MeshDimension<D>(sizes)};
+            std::vector<double> comp(static_cast<std::size_t>(count));
+            for (std::size_t c = 0; c < C; ++c) {
+                detail::check(intp_spline_download_ctrl(handle(c), comp.data(), count));
+                auto it = mesh.begin();
+                for (std::int64_t i = 0; i < count; ++i, ++it)
+                    traits::set(*it, c, static_cast<typename traits::scalar>(comp[static_cast<std::size_t>(i)]));
+            }
+            host_->ctrl = std::move(mesh);
+        });
+        return *host_;
+    }
+};
 
 // ---- InterpolationFunction -----------------------------------------------
 
-template <typename T, std::size_t D, std::size_t O, typename U = double>
+template <typename T, std::size_t D, std::size_t O, typename U>
 class InterpolationFunction {
-    static_assert(std::is_same<T, double>::value && std::is_same<U, double>::value,
-                  "libintp_amd evaluates FP64 splines: T and U must be double");
+    using traits = detail::value_traits<T>;
+    using scalar_type = typename traits::scalar;
+    static constexpr std::size_t C = traits::components;
+    static_assert(std::is_same<scalar_type, double>::value || std::is_same<scalar_type, float>::value,
+                  "values (or their components) must be double or float");
+    static_assert(std::is_same<U, double>::value || std::is_same<U, float>::value,
+                  "coordinates must be double or float");
     static_assert(D >= 1 && D <= INTP_MAX_DIM, "dimension without compiled kernels");
     static_assert(O <= INTP_MAX_ORDER, "order without compiled kernels");
 
@@ -271,10 +447,16 @@ class InterpolationFunction {
     constexpr static size_type dim = D;
     constexpr static size_type order = O;
     using function_type = InterpolationFunction<T, D, O, U>;
+    using spline_type = BSpline<T, D, O, U>;
     template <typename V>
     using DimArray = std::array<V, D>;
 
     friend class InterpolationFunctionTemplate<T, D, O, U>;
+
+    // FP32 values (or components) are built and evaluated in the
+    // refined-FP32 form: FP32 control points solved with FP64-residual
+    // refinement (intp_build_refined, this many sweeps).
+    static constexpr std::int32_t refine_sweeps = 1;
 
     // 1D: (periodic, values range, (xmin, xmax)), Interpolation.hpp:49-79
     template <typename InputIter, typename C1, typename C2,
@@ -296,9 +478,22 @@ class InterpolationFunction {
         intp_grid_desc desc = describe(periodicity, f_mesh.dimension());
         std::vector<std::vector<double>> keep(D);
         detail::describe_all<D>(desc, keep, std::index_sequence_for<Ts...>{}, x_ranges...);
-        intp_spline s = nullptr;
-        detail::check(intp_build(detail::context(), &desc, f_mesh.data(), INTP_MEM_HOST, &s, nullptr));
-        adopt(detail::own(s));
+        std::array<detail::SplineHandle, C> splines;
+        if (C == 1) {
+            const std::vector<double> values = component(f_mesh, 0);
+            intp_spline s = nullptr;
+            detail::check(refined() ? intp_build_refined(detail::context(), &desc, values.data(), INTP_MEM_HOST,
+                                                         refine_sweeps, &s, nullptr)
+                                    : intp_build(detail::context(), &desc, values.data(), INTP_MEM_HOST, &s, nullptr));
+            splines[0] = detail::own(s);
+        } else {
+            // components share one grid: one template, one solve each
+            intp_template t = nullptr;
+            detail::check(intp_template_create(detail::context(), &desc, &t));
+            const detail::TemplateHandle th(t, [](intp_template p) { intp_template_destroy(p); });
+            for (std::size_t c = 0; c < C; ++c) splines[c] = interpolate_component(th.get(), f_mesh, c);
+        }
+        adopt(splines);
     }
 
     // all dimensions clamped, :104-107
@@ -310,6 +505,8 @@ class InterpolationFunction {
     InterpolationFunction() = default;
 
     // ---- evaluation (Interpolation.hpp:132-244) ------------------------
+    // The scalar members evaluate on the calling thread from a host copy
+    // of the control points (made on the first call), like the reference.
     template <typename... Coords, typename = typename std::enable_if<
                                       std::is_arithmetic<typename std::common_type<Coords...>::type>::value>::type>
     val_type operator()(Coords... x) const {
@@ -351,57 +548,70 @@ class InterpolationFunction {
                              DimArray<size_type>{static_cast<size_type>(coord_deri_order.second)...});
     }
 
-    // ---- batched evaluation (extension) --------------------------------
+    // ---- batched evaluation (extension; scalar double values) ----------
     // npts points, coordinate d of point i at coords[i * D + d] (AoS);
     // output o of point i at out[o * npts + i]; derivative orders
     // orders[o * D + d] (nullptr: the value only).  bounds_check: the at()
-    // semantics.  Host buffers; one synchronous call.
-    void eval(size_type npts, const coord_type* coords, val_type* out, size_type nout = 1,
+    // semantics.  Host buffers; one synchronous call on the GPU.
+    void eval(size_type npts, const double* coords, double* out, size_type nout = 1,
               const size_type* orders = nullptr, bool bounds_check = false) const {
         eval_raw(npts, coords, 1, static_cast<std::int64_t>(D), out, static_cast<std::int64_t>(npts), 1, nout,
                  orders, INTP_EVAL_HOST_MEM | (bounds_check ? INTP_EVAL_BOUNDS_CHECK : 0u), nullptr);
     }
     // Device buffers with explicit strides (see intp_eval), asynchronous
     // on `stream` (a hipStream_t) unless bounds_check is set.
-    void eval_device(size_type npts, const coord_type* coords, std::int64_t cs_dim, std::int64_t cs_pt,
-                     val_type* out, std::int64_t os_out, std::int64_t os_pt, size_type nout = 1,
-                     const size_type* orders = nullptr, bool bounds_check = false, void* stream = nullptr) const {
+    void eval_device(size_type npts, const double* coords, std::int64_t cs_dim, std::int64_t cs_pt, double* out,
+                     std::int64_t os_out, std::int64_t os_pt, size_type nout = 1, const size_type* orders = nullptr,
+                     bool bounds_check = false, void* stream = nullptr) const {
         eval_raw(npts, coords, cs_dim, cs_pt, out, os_out, os_pt, nout, orders,
                  bounds_check ? INTP_EVAL_BOUNDS_CHECK : 0u, stream);
     }
 
     // ---- properties (Interpolation.hpp:248-267) ------------------------
-    bool periodicity(size_type d) const { return info().periodic[d] != 0; }
-    bool uniform(size_type d) const { return info().uniform[d] != 0; }
-    const std::pair<coord_type, coord_type>& range(size_type d) const { return ranges_[d]; }
+    bool periodicity(size_type d) const { return spline_.info().periodic[d] != 0; }
+    bool uniform(size_type d) const { return spline_.info().uniform[d] != 0; }
+    const std::pair<coord_type, coord_type>& range(size_type d) const {
+        (void)spline_.handle();
+        return spline_.range(d);
+    }
+    const spline_type& spline() const {
+        (void)spline_.handle();
+        return spline_;
+    }
     static constexpr size_type get_order() { return order; }
 
-    // Control points (C-order) and knot vectors, copied to the host.
-    std::vector<val_type> control_points() const {
-        const intp_spline_info& in = info();
+    // Control points (C-order, scalar values: component 0) and knot
+    // vectors, copied to the host.
+    std::vector<double> control_points(std::size_t component = 0) const {
+        const intp_spline_info& in = spline_.info();
         std::int64_t count = 1;
         for (size_type d = 0; d < D; ++d) count *= in.n[d];
-        std::vector<val_type> out(static_cast<std::size_t>(count));
-        detail::check(intp_spline_download_ctrl(handle(), out.data(), count));
+        std::vector<double> out(static_cast<std::size_t>(count));
+        detail::check(intp_spline_download_ctrl(spline_.handle(component), out.data(), count));
         return out;
     }
-    std::vector<coord_type> knots(size_type d) const {
-        std::vector<coord_type> out(static_cast<std::size_t>(info().nknots[d]));
-        detail::check(intp_spline_download_knots(handle(), static_cast<std::int32_t>(d), out.data(),
+    std::vector<double> knots(size_type d) const {
+        std::vector<double> out(static_cast<std::size_t>(spline_.info().nknots[d]));
+        detail::check(intp_spline_download_knots(spline_.handle(), static_cast<std::int32_t>(d), out.data(),
                                                  static_cast<std::int64_t>(out.size())));
         return out;
     }
 
     // Value at a fixed coordinate of any function on the same grid
-    // (Interpolation.hpp:493-506): a callable taking a function_type.
+    // (Interpolation.hpp:493-506; BSpline.hpp:244-297): the knot spans and
+    // basis values are computed here, once; the callable applies them to
+    // a function's control points (a host proxy, intp_proxy_create).
     auto eval_proxy(DimArray<coord_type> coords) const {
-        return [coords](const function_type& f) { return f(coords); };
+        return make_proxy(nullptr, spline_.handle(), coords);
     }
 
+    // the library handle of component c (extension)
+    intp_spline handle(std::size_t c = 0) const { return spline_.handle(c); }
+
    private:
-    detail::SplineHandle spline_;
-    intp_spline_info info_{};
-    DimArray<std::pair<coord_type, coord_type>> ranges_{};
+    spline_type spline_;
+
+    static constexpr bool refined() { return std::is_same<scalar_type, float>::value; }
 
     static intp_grid_desc describe(const DimArray<bool>& periodicity, const MeshDimension<D>& dims) {
         intp_grid_desc desc{};
@@ -414,36 +624,62 @@ class InterpolationFunction {
         return desc;
     }
 
-    void adopt(detail::SplineHandle s) {
-        spline_ = std::move(s);
-        detail::check(intp_spline_get_info(spline_.get(), &info_));
-        for (size_type d = 0; d < D; ++d) ranges_[d] = std::make_pair(info_.range_lo[d], info_.range_hi[d]);
+    // Component c of a value mesh, as doubles (C order).
+    static std::vector<double> component(const Mesh<T, D>& mesh, std::size_t c) {
+        std::vector<double> v;
+        v.reserve(mesh.size());
+        for (auto it = mesh.begin(); it != mesh.end(); ++it) v.push_back(static_cast<double>(traits::get(*it, c)));
+        return v;
     }
 
-    intp_spline handle() const {
-        if (!spline_) throw std::runtime_error("empty InterpolationFunction");
-        return spline_.get();
-    }
-    const intp_spline_info& info() const {
-        (void)handle();
-        return info_;
+    static detail::SplineHandle interpolate_component(intp_template t, const Mesh<T, D>& mesh, std::size_t c) {
+        const std::vector<double> values = component(mesh, c);
+        intp_spline s = nullptr;
+        detail::check(refined() ? intp_interpolate_refined(t, values.data(), INTP_MEM_HOST, refine_sweeps, &s, nullptr)
+                                : intp_interpolate(t, values.data(), INTP_MEM_HOST, &s, nullptr));
+        return detail::own(s);
     }
 
-    void eval_raw(size_type npts, const coord_type* coords, std::int64_t cs_dim, std::int64_t cs_pt,
-                  val_type* out, std::int64_t os_out, std::int64_t os_pt, size_type nout, const size_type* orders,
+    void adopt(const std::array<detail::SplineHandle, C>& splines) {
+        intp_spline_info info{};
+        detail::check(intp_spline_get_info(splines[0].get(), &info));
+        spline_.bind(splines, info);
+    }
+
+    void eval_raw(size_type npts, const double* coords, std::int64_t cs_dim, std::int64_t cs_pt, double* out,
+                  std::int64_t os_out, std::int64_t os_pt, size_type nout, const size_type* orders,
                   std::uint32_t flags, void* stream) const {
+        static_assert(C == 1, "batched evaluation takes scalar values");
         std::vector<std::int32_t> r(nout * D, 0);
         if (orders)
             for (size_type i = 0; i < nout * D; ++i) r[i] = static_cast<std::int32_t>(orders[i]);
-        detail::check(intp_eval(handle(), static_cast<std::int64_t>(npts), coords, cs_dim, cs_pt,
+        detail::check(intp_eval(spline_.handle(), static_cast<std::int64_t>(npts), coords, cs_dim, cs_pt,
                                 static_cast<std::int32_t>(nout), r.data(), out, os_out, os_pt, flags, stream));
     }
 
     val_type eval_one(const DimArray<coord_type>& coord, const DimArray<size_type>& r, bool bounds) const {
-        val_type v = 0;
-        eval_raw(1, coord.data(), 1, static_cast<std::int64_t>(D), &v, 1, 1, 1, r.data(),
-                 INTP_EVAL_HOST_MEM | (bounds ? INTP_EVAL_BOUNDS_CHECK : 0u), nullptr);
-        return v;
+        DimArray<double> x;
+        for (size_type d = 0; d < D; ++d) x[d] = static_cast<double>(coord[d]);
+        return spline_.eval(x, r, bounds);
+    }
+
+    // A host proxy of one coordinate on the grid of template t or spline s.
+    static auto make_proxy(intp_template t, intp_spline s, const DimArray<coord_type>& coord) {
+        DimArray<double> x;
+        for (size_type d = 0; d < D; ++d) x[d] = static_cast<double>(coord[d]);
+        intp_proxy p = nullptr;
+        detail::check(intp_proxy_create(t, s, 1, x.data(), 1, static_cast<std::int64_t>(D), INTP_EVAL_HOST_MEM, &p,
+                                        nullptr));
+        const detail::ProxyHandle ph = detail::own(p);
+        return [ph](const function_type& f) {
+            val_type v{};
+            for (std::size_t c = 0; c < C; ++c) {
+                double out = 0;
+                detail::check(intp_proxy_eval(ph.get(), f.handle(c), &out, 1, 0u, nullptr));
+                traits::set(v, c, static_cast<scalar_type>(out));
+            }
+            return v;
+        };
     }
 };
 
@@ -497,10 +733,11 @@ class InterpolationFunctionTemplate {
         for (size_type d = 0; d < D; ++d)
             if (mesh.dim_size(d) != dimension_.dim_size(d))
                 throw std::runtime_error("value mesh shape does not match the template");
-        intp_spline s = nullptr;
-        detail::check(intp_interpolate(template_.get(), mesh.data(), INTP_MEM_HOST, &s, nullptr));
+        std::array<detail::SplineHandle, detail::value_traits<T>::components> splines;
+        for (std::size_t c = 0; c < splines.size(); ++c)
+            splines[c] = function_type::interpolate_component(template_.get(), mesh, c);
         function_type f;
-        f.adopt(detail::own(s));
+        f.adopt(splines);
         return f;
     }
     template <typename It, typename = typename std::enable_if<D == 1 && detail::is_iterator<It>::value>::type>
@@ -511,25 +748,25 @@ class InterpolationFunctionTemplate {
     void interpolate(function_type& interp, MeshOrIterPair&& mesh_or_range) const {
         interp = interpolate(std::forward<MeshOrIterPair>(mesh_or_range));
     }
-    // Device-resident values (C-order doubles), asynchronous on `stream`.
-    function_type interpolate_device(const val_type* values, void* stream = nullptr) const {
+    // Device-resident values (C-order doubles, scalar double splines),
+    // asynchronous on `stream`.
+    function_type interpolate_device(const double* values, void* stream = nullptr) const {
+        static_assert(detail::value_traits<T>::components == 1, "device values are scalar");
         intp_spline s = nullptr;
         detail::check(intp_interpolate(template_.get(), values, INTP_MEM_DEVICE, &s, stream));
         function_type f;
-        f.adopt(detail::own(s));
+        f.adopt({detail::own(s)});
         return f;
     }
 
     // value at a fixed coordinate of any function interpolated on this
-    // grid, :145-176
+    // grid, :145-176: spans and basis computed once (a host proxy)
     template <typename... Coords, typename = typename std::enable_if<
                                       std::is_arithmetic<typename std::common_type<Coords...>::type>::value>::type>
     auto eval_proxy(Coords... x) const {
         return eval_proxy(DimArray<coord_type>{static_cast<coord_type>(x)...});
     }
-    auto eval_proxy(DimArray<coord_type> coord) const {
-        return [coord](const function_type& f) { return f(coord); };
-    }
+    auto eval_proxy(DimArray<coord_type> coord) const { return function_type::make_proxy(template_.get(), nullptr, coord); }
     using eval_proxy_t = decltype(std::declval<function_type>().eval_proxy(DimArray<coord_type>{}));
 
    private:

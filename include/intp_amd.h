@@ -62,10 +62,24 @@ typedef enum {
                                      read back with intp_eval_timings()   */
 #define INTP_EVAL_OUT_F32 16u     /* `out` points to float storage: results
                                      are stored as FP32 (refined splines) */
+#define INTP_EVAL_GROUP_TWO_PASS 128u /* group with two radix passes over
+                                     small bricks (24 KB patches) even where
+                                     the one-pass grouping over large bricks
+                                     applies                               */
+#define INTP_EVAL_HOST_MIRROR 64u /* evaluate on the calling host thread from
+                                     a host copy of the control points
+                                     (downloaded on the first such call and
+                                     kept with the spline); coords and out
+                                     are host memory, the call is
+                                     synchronous and launches nothing.  The
+                                     scalar members of the C++ drop-in use
+                                     it: one point per call, where a device
+                                     round trip costs tens of microseconds */
 
 typedef struct intp_ctx_s* intp_ctx;
 typedef struct intp_template_s* intp_template;
 typedef struct intp_spline_s* intp_spline;
+typedef struct intp_proxy_s* intp_proxy;
 
 /* Grid description: one InterpolationFunctionTemplate's arguments
  * (InterpolationTemplate.hpp:60-79; Interpolation.hpp:92-107).
@@ -177,13 +191,42 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
  * INTP_E_DOMAIN (and synchronizes `stream`) if any clamped coordinate is
  * outside range(d).  Orders above the spline order give 0.
  * Large batches on uniform 2D/3D grids are first grouped by brick of the
- * control-point table (two counting-sort passes in scratch device memory
- * from the stream-ordered allocator) so that each workgroup gathers from a
- * cache-resident patch; results are written to the caller's positions and
- * do not depend on the grouping.                                          */
+ * control-point table so that each workgroup gathers from a patch of the
+ * table staged in LDS: one counting pass and one scatter of 32-byte
+ * records when the grid has at most 16384 bricks whose patches fit 64 KB,
+ * else two radix passes over smaller bricks (scratch device memory from
+ * the stream-ordered allocator, about 32 bytes per point plus tables).
+ * Results are written to the caller's positions and do not depend on the
+ * grouping.                                                              */
 int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
               int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
               uint32_t flags, void* stream);
+
+/* With INTP_EVAL_BOUNDS_CHECK, INTP_E_DOMAIN names the first offending
+ * point (index, dimension, coordinate and range) in intp_last_error();
+ * the outputs of a batch that fails the check are unspecified.           */
+
+/* ---- evaluation proxies ------------------------------------------------ */
+/* Replaces eval_proxy / pre_calc_coef (Interpolation.hpp:493-506,
+ * InterpolationTemplate.hpp:145-176, BSpline.hpp:244-297), for many points
+ * at once: the knot spans and value basis of `npts` fixed points on the
+ * grid of template `t` or of spline `s` (exactly one of them) are computed
+ * once; intp_proxy_eval then gives each point's value for any spline on an
+ * equal grid with one gather and dot product per point, no span search and
+ * no basis evaluation.
+ *   flags & INTP_EVAL_HOST_MEM: a host proxy (coords host memory; applied
+ *     on the calling thread to the spline's host copy of its control
+ *     points, out host memory): the reference's per-point proxy;
+ *   otherwise a device proxy (coords device memory, asynchronous on
+ *     `stream`; applied by a kernel, out device memory, FP32 with
+ *     INTP_EVAL_OUT_F32): its values equal intp_eval's bit for bit.
+ *   flags & INTP_EVAL_BOUNDS_CHECK: at() semantics at creation.
+ * Output of point i: out[i * os_pt].                                      */
+int intp_proxy_create(intp_template t, intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
+                      int64_t cs_pt, uint32_t flags, intp_proxy* out, void* stream);
+int intp_proxy_eval(intp_proxy p, intp_spline s, double* out, int64_t os_pt, uint32_t flags, void* stream);
+int intp_proxy_points(intp_proxy p, int64_t* npts);
+int intp_proxy_destroy(intp_proxy p);
 
 /* ---- batches: many independent splines built and evaluated together -- */
 /* No single reference counterpart: a batch is N independent

@@ -6,6 +6,7 @@
 // exits non-zero on any failure.
 #include <intp/Interpolation.hpp>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -146,13 +147,94 @@ void surface_checks() {
     }
     const std::size_t orders[6] = {0, 0, 1, 0, 0, 1};
     fn.eval(1000, pts.data(), out.data(), 3, orders);
-    bool same = true;
+    // batched calls run on the GPU (per-class basis polynomials), scalar
+    // calls on the host copy (Cox-de Boor): equal to round-off
+    double worst = 0;
     for (int i = 0; i < 1000; i += 37) {
         const std::array<double, 2> x{pts[2 * i], pts[2 * i + 1]};
-        same = same && out[i] == fn(x) && out[1000 + i] == fn.derivative(x, 1, 0) &&
-               out[2000 + i] == fn.derivative(x, 0, 1);
+        const double s[3] = {fn(x), fn.derivative(x, 1, 0), fn.derivative(x, 0, 1)};
+        for (int o = 0; o < 3; ++o)
+            worst = std::max(worst, std::fabs(out[o * 1000 + i] - s[o]) / std::max(1.0, std::fabs(s[o])));
     }
-    expect("batched outputs == scalar calls", same);
+    std::printf("batched vs scalar: max relative difference %.3e\n", worst);
+    expect("batched outputs == scalar calls (1e-13)", worst < 1e-13);
+
+    // spline(): read-only view of the BSpline (Interpolation.hpp:264)
+    const auto& sp = fn.spline();
+    const std::vector<double> k0 = fn.knots(0);
+    expect("spline() knots", sp.knots_num(0) == k0.size() && std::equal(sp.knots_begin(0), sp.knots_end(0), k0.begin()) &&
+                                 *(sp.knots_end(1) - 1) == fn.knots(1).back());
+    const std::vector<double> cp = fn.control_points();
+    expect("spline() control points", sp.control_points().size() == cp.size() &&
+                                          std::equal(sp.control_points().begin(), sp.control_points().end(), cp.begin()) &&
+                                          sp.control_points().dim_size(0) == 24);
+    expect("spline() properties", sp.periodicity(1) && !sp.periodicity(0) && sp.get_order() == 3 &&
+                                      sp.range(1).second == fn.range(1).second);
+    expect("spline() evaluation", sp(0.3, 2.0) == fn(0.3, 2.0) &&
+                                      sp.derivative_at({std::make_pair(0.3, std::size_t{1}), std::make_pair(2.0, std::size_t{2})}) ==
+                                          fn.derivative({0.3, 2.0}, 1, 2));
+}
+
+// A fixed-size vector value type, as the reference's Vec<D, T>
+// (test/src/include/Vec.hpp): static dim, value_type, operator[].
+struct Vec2f {
+    static constexpr std::size_t dim = 2;
+    using value_type = float;
+    std::array<float, 2> c{};
+    Vec2f() = default;
+    Vec2f(double x, double y) : c{static_cast<float>(x), static_cast<float>(y)} {}
+    float& operator[](std::size_t i) { return c[i]; }
+    const float& operator[](std::size_t i) const { return c[i]; }
+    float mag() const { return std::sqrt(c[0] * c[0] + c[1] * c[1]); }
+};
+
+void value_type_checks() {
+    // float values and coordinates: the refined-FP32 form
+    intp::Mesh<float, 2> f(30, 26);
+    intp::Mesh<double, 2> g(30, 26);
+    for (std::size_t i = 0; i < 30; ++i)
+        for (std::size_t j = 0; j < 26; ++j) {
+            const double x = i / 29.0, y = j / 25.0;
+            g(i, j) = std::sin(3.0 * x) * std::cos(2.0 * y) + x * y;
+            f(i, j) = static_cast<float>(g(i, j));
+        }
+    const intp::InterpolationFunction<float, 2, 3, float> ff(f, std::make_pair(0.0f, 1.0f), std::make_pair(0.0f, 1.0f));
+    const intp::InterpolationFunction<double, 2, 3> fd(g, std::make_pair(0.0, 1.0), std::make_pair(0.0, 1.0));
+    double worst = 0;
+    for (int i = 0; i < 200; ++i) {
+        const float x = static_cast<float>((i * 37 % 200) / 199.0), y = static_cast<float>(i / 199.0);
+        worst = std::max(worst, std::fabs(static_cast<double>(ff(x, y)) - fd(static_cast<double>(x), static_cast<double>(y))));
+        worst = std::max(worst, std::fabs(static_cast<double>(ff.derivative({x, y}, 1, 0)) -
+                                          fd.derivative({static_cast<double>(x), static_cast<double>(y)}, 1, 0)) / 10.0);
+    }
+    std::printf("float function vs double function: max difference %.3e\n", worst);
+    expect("InterpolationFunction<float, 2, 3, float> (1e-5)", worst < 1e-5);
+
+    // 2D points on a circle, periodic, vector-valued
+    // (interpolation-test.cpp:674-703, tolerance 1e-5)
+    std::vector<Vec2f> pts;
+    const std::size_t pts_n = 31;
+    for (std::size_t i = 0; i < pts_n; ++i) {
+        const double theta = 2. * M_PI * static_cast<double>(i) / pts_n;
+        pts.emplace_back(std::cos(theta), std::sin(theta));
+    }
+    const intp::InterpolationFunction1D<3, Vec2f, float> circle(intp::util::get_range(pts), true);
+    const std::size_t sample_n = 1024;
+    float err = 0;
+    for (std::size_t i = 0; i < sample_n; ++i) {
+        const float theta = 2.f * static_cast<float>(M_PI) * static_cast<float>(i) / sample_n;
+        err += std::abs(circle(theta).mag() - 1.f);
+    }
+    err /= sample_n;
+    std::printf("circle of Vec2f: mean |mag - 1| = %.3e\n", err);
+    expect("InterpolationFunction1D<3, Vec2f, float> circle (1e-5)", err < 1.e-5f);
+    // the vector spline's components are the scalar splines of the components
+    std::vector<float> xs;
+    for (const auto& p : pts) xs.push_back(p[0]);
+    const intp::InterpolationFunction1D<3, float, float> cx(intp::util::get_range(xs), true);
+    expect("vector components == scalar splines", circle(1.3f)[0] == cx(1.3f));
+    const auto proxy = intp::InterpolationFunctionTemplate1D<3, Vec2f, float>(pts_n, true).eval_proxy(2.5f);
+    (void)proxy;
 }
 
 }  // namespace
@@ -160,6 +242,7 @@ void surface_checks() {
 int main() {
 #include "mma_cases.inc"
     surface_checks();
+    value_type_checks();
     std::printf("%d of %d checks failed\n", failures, checks);
     return failures == 0 ? 0 : 1;
 }

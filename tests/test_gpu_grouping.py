@@ -23,10 +23,10 @@ def amd():
     return amd
 
 
-def _eval_both(g, q, orders):
+def _eval_both(g, q, orders, method=True):
     import torch
     qd = torch.from_numpy(q).cuda()
-    a = g.eval(qd, orders, group=True).cpu().numpy()
+    a = g.eval(qd, orders, group=method).cpu().numpy()
     # same call, caller order
     import ctypes as C
     from bsplineinterpolation_amd import _lib
@@ -41,13 +41,18 @@ def _eval_both(g, q, orders):
     return a, out.cpu().numpy()
 
 
+# method True: one pass over large bricks (brick items) where the grid
+# allows it; "two_pass": two radix passes over small bricks
+@pytest.mark.parametrize("method", [True, "two_pass"])
 @pytest.mark.parametrize("shape,k,per,npts", [
     ((96, 80, 64), 3, [True, False, False], (1 << 21) + 3),
     ((40, 33, 50), 5, [False, True, True], (1 << 20)),
     ((300, 200), 4, [True, True], (1 << 20) + 77),
     ((257, 129), 3, [False, False], (1 << 20)),
+    ((17, 9, 300), 2, [True, True, False], (1 << 20) + 1),
+    ((1000, 8), 7, [False, True], (1 << 20)),
 ])
-def test_gpu_grouped_equals_ungrouped(amd, orc, shape, k, per, npts):
+def test_gpu_grouped_equals_ungrouped(amd, orc, shape, k, per, npts, method):
     rng = np.random.default_rng(sum(shape) + k)
     f = rng.standard_normal(shape)
     D = len(shape)
@@ -57,7 +62,7 @@ def test_gpu_grouped_equals_ungrouped(amd, orc, shape, k, per, npts):
     lo = np.array([r[0] for r in rg])
     q = (lo[:, None] + span[:, None] * rng.uniform(-0.1, 1.1, (D, npts)))
     orders = [[0] * D] + [list(e) for e in np.eye(D, dtype=int)] + [[2] * D]
-    grouped, plain = _eval_both(g, q, orders)
+    grouped, plain = _eval_both(g, q, orders, method)
     assert np.array_equal(grouped, plain)
     o = orc.OracleSpline(k, f, periodic=per, ranges=rg)
     idx = rng.choice(npts, 4096, replace=False)
@@ -93,6 +98,25 @@ def test_gpu_grouped_host_buffers_and_bounds(amd):
     assert rc == _lib.INTP_E_INVALID
 
 
+def test_gpu_grouped_skewed_points(amd):
+    """Most points in one brick (many brick items of one bin, most bins
+    empty), the rest spread out, outside the range on clamped dimensions
+    and wrapped on the periodic one: grouped results equal caller order."""
+    rng = np.random.default_rng(77)
+    f = rng.standard_normal((80, 70, 60))
+    g = amd.InterpolationFunction(3, f, periodic=[True, False, False], ranges=[(0, 1), (0, 1), (0, 1)])
+    n = (1 << 20) + 333
+    q = np.empty((3, n))
+    hot = n - 50_000
+    q[:, :hot] = 0.41 + 0.01 * rng.random((3, hot))
+    q[:, hot:] = rng.uniform(-0.3, 1.3, (3, n - hot))
+    q[0, hot: hot + 100] += 7.0
+    orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    for method in (True, "two_pass"):
+        grouped, plain = _eval_both(g, q, orders, method)
+        assert np.array_equal(grouped, plain), method
+
+
 @pytest.mark.parametrize("npts", [1, 7, 1000, 2049])
 def test_gpu_forced_grouping_of_small_batches(amd, npts):
     """INTP_EVAL_GROUPED on batches far below the automatic threshold
@@ -106,4 +130,5 @@ def test_gpu_forced_grouping_of_small_batches(amd, npts):
     orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
     a = g.eval(q, orders, group=True).cpu().numpy()
     b = g.eval(q, orders, group=False).cpu().numpy()
-    assert np.array_equal(a, b)
+    c = g.eval(q, orders, group="two_pass").cpu().numpy()
+    assert np.array_equal(a, b) and np.array_equal(c, b)
