@@ -244,8 +244,8 @@ class InterpolationFunction:
         float32 when ``out`` is float32 or ``out_dtype`` says so.  ``timed``
         records device events around the call (read with eval_timings()).
         ``group``: True / False forces / forbids grouping the points by
-        brick (INTP_EVAL_GROUPED / INTP_EVAL_UNGROUPED); "two_pass" forces
-        the two-radix-pass grouping (INTP_EVAL_GROUP_TWO_PASS); None lets
+        brick (INTP_EVAL_GROUPED / INTP_EVAL_UNGROUPED); "one_pass" forces
+        the one-pass grouping (INTP_EVAL_GROUP_ONE_PASS); None lets
         the library decide from the batch and table sizes.  Results are the
         same either way."""
         lib = _lib.load()
@@ -254,8 +254,8 @@ class InterpolationFunction:
             np.ascontiguousarray(np.asarray(orders, dtype=np.int32).reshape(-1, D))
         nout = orders.shape[0]
         flags = (_lib.INTP_EVAL_BOUNDS_CHECK if bounds else 0) | (_lib.INTP_EVAL_TIMED if timed else 0)
-        if group == "two_pass":
-            flags |= _lib.INTP_EVAL_GROUPED | _lib.INTP_EVAL_GROUP_TWO_PASS
+        if group == "one_pass":
+            flags |= _lib.INTP_EVAL_GROUPED | _lib.INTP_EVAL_GROUP_ONE_PASS
         elif group is not None:
             flags |= _lib.INTP_EVAL_GROUPED if group else _lib.INTP_EVAL_UNGROUPED
         op = orders.ctypes.data_as(C.POINTER(C.c_int32))

@@ -31,7 +31,7 @@ INTP_EVAL_TIMED = 8
 INTP_EVAL_OUT_F32 = 16
 INTP_EVAL_GROUPED = 32
 INTP_EVAL_HOST_MIRROR = 64
-INTP_EVAL_GROUP_TWO_PASS = 128
+INTP_EVAL_GROUP_ONE_PASS = 128
 INTP_BATCH_GRAPH = 1
 INTP_BATCH_AUTOTUNE = 2
 INTP_BATCH_SPLIT_LINES = 4

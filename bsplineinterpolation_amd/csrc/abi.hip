@@ -134,7 +134,7 @@ int upload_grid(GridData& g, hipStream_t stream) {
     g.image.assign(total, 0);
     for (size_t i = 0; i < items.size(); ++i)
         if (items[i].bytes) std::memcpy(g.image.data() + offset[i], items[i].src, items[i].bytes);
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&g.dev), total, stream), "hipMallocAsync (grid tables)");
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&g.dev), total, stream), "device allocation (grid tables)");
     HIP_TRY(hipMemcpyAsync(g.dev, g.image.data(), total, hipMemcpyHostToDevice, stream), "hipMemcpyAsync");
     HIP_TRY(g.uses.record(stream), "hipEventRecord");
     HIP_TRY(hipEventCreateWithFlags(&g.uploaded, hipEventDisableTiming), "hipEventCreate");
@@ -661,15 +661,15 @@ int interpolate_impl(intp_template t, const double* values, int values_mem, bool
     s->grid = t->grid;
     s->ctrl_bytes = refined ? 4 : 8;
     s->sweeps = refined ? sweeps : -1;
-    HIP_TRY(hipMallocAsync(&s->ctrl, static_cast<size_t>(g.total) * s->ctrl_bytes, stream),
-            "hipMallocAsync (control points)");
+    HIP_TRY(device_alloc(&s->ctrl, static_cast<size_t>(g.total) * s->ctrl_bytes, stream),
+            "device allocation (control points)");
     const double* src = values;
     double* staged = nullptr;
     unsigned char* scratch = nullptr;
     int rc = INTP_OK;
     hipError_t e = hipSuccess;
     if (values_mem == INTP_MEM_HOST) {
-        e = hipMallocAsync(reinterpret_cast<void**>(&staged), values_bytes, stream);
+        e = device_alloc(reinterpret_cast<void**>(&staged), values_bytes, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(staged, values, values_bytes, hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) rc = hip_fail(e, "staging values");
         src = staged;
@@ -677,8 +677,8 @@ int interpolate_impl(intp_template t, const double* values, int values_mem, bool
     if (rc == INTP_OK && refined && sweeps > 0) {
         const size_t bytes = ((static_cast<size_t>(g.total) * 4 + 255) / 256) * 256 +
                              2 * ((static_cast<size_t>(g.total) + 31) / 32) * 32 * sizeof(double);
-        e = hipMallocAsync(reinterpret_cast<void**>(&scratch), bytes, stream);
-        if (e != hipSuccess) rc = hip_fail(e, "hipMallocAsync (refinement scratch)");
+        e = device_alloc(reinterpret_cast<void**>(&scratch), bytes, stream);
+        if (e != hipSuccess) rc = hip_fail(e, "device allocation (refinement scratch)");
     }
     if (rc == INTP_OK)
         rc = refined ? refine_into(g, src, static_cast<float*>(s->ctrl), sweeps, scratch, stream)
@@ -755,7 +755,7 @@ int intp_spline_from_knots(intp_ctx ctx, int32_t dim, int32_t order, const int32
     std::unique_ptr<intp_spline_s> s(new intp_spline_s());
     s->grid = std::move(g);
     const size_t bytes = static_cast<size_t>(s->grid->total) * sizeof(double);
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&s->ctrl), bytes, rt->setup), "hipMallocAsync (control points)");
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&s->ctrl), bytes, rt->setup), "device allocation (control points)");
     HIP_TRY(hipMemcpyAsync(s->ctrl, ctrl, bytes, hipMemcpyHostToDevice, rt->setup), "hipMemcpyAsync");
     HIP_TRY(s->uses.record(rt->setup), "hipEventRecord");
     HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
@@ -891,7 +891,7 @@ struct Scratch {
     bool fixed() const { return base != nullptr; }
     bool fits(size_t bytes) const { return !fixed() || (used + 255) / 256 * 256 + bytes <= cap; }
     hipError_t get(size_t bytes, void** out) {
-        if (!fixed()) return hipMallocAsync(out, bytes, stream);
+        if (!fixed()) return device_alloc(out, bytes, stream);
         const size_t at = (used + 255) / 256 * 256;
         if (at + bytes > cap) return hipErrorOutOfMemory;
         *out = base + at;
@@ -981,8 +981,8 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         sc.put(dbad);
     };
     if (host) {
-        hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&dcoords), cext * sizeof(double), stream);
-        if (e == hipSuccess) e = hipMallocAsync(&dout, oext * out_elem, stream);
+        hipError_t e = device_alloc(reinterpret_cast<void**>(&dcoords), cext * sizeof(double), stream);
+        if (e == hipSuccess) e = device_alloc(&dout, oext * out_elem, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dcoords, coords, cext * sizeof(double), hipMemcpyHostToDevice, stream);
         if (e != hipSuccess) {
             cleanup();
@@ -1074,9 +1074,14 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     BrickGrid ib{};
     int64_t item_patch = 0;
     int32_t item_bins = 0;
-    bool one_pass = fast && D >= 2 && group && npts <= 0xFFFFFFFFll && !(flags & INTP_EVAL_GROUP_TWO_PASS) &&
-                    item_bricks(g, s->ctrl_bytes, &ib, &item_patch, &item_bins);
-    bool two_pass = !one_pass && fast && D >= 2 && group && npts <= 0xFFFFFFFFll && bin_scan_tiles(npts) <= 4096;
+    // Two radix passes by default (measured faster at 256^3); one pass on
+    // request, and for a fixed scratch buffer, where it needs half the
+    // memory.
+    const bool groupable = fast && D >= 2 && group && npts <= 0xFFFFFFFFll;
+    const bool bricks_ok = groupable && item_bricks(g, s->ctrl_bytes, &ib, &item_patch, &item_bins);
+    const bool two_ok = groupable && bin_scan_tiles(npts) <= 4096;
+    bool one_pass = bricks_ok && ((flags & INTP_EVAL_GROUP_ONE_PASS) || sc.fixed() || !two_ok);
+    bool two_pass = !one_pass && two_ok;
     if (sc.fixed() && (one_pass || two_pass)) {
         // the grouping's scratch must fit the fixed buffer; otherwise the
         // points are evaluated in caller order (same results), unless the
@@ -1348,7 +1353,7 @@ int intp_proxy_create(intp_template t, intp_spline s, int64_t npts, const double
     if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
     const size_t first_bytes = (static_cast<size_t>(npts * D) * sizeof(int32_t) + 255) / 256 * 256;
     const size_t bytes = first_bytes + static_cast<size_t>(npts * D * w) * sizeof(double) + 256;
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&p->dev), bytes, stream), "hipMallocAsync (proxy)");
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&p->dev), bytes, stream), "device allocation (proxy)");
     p->first_d = reinterpret_cast<int32_t*>(p->dev);
     p->weights_d = reinterpret_cast<double*>(p->dev + first_bytes);
     p->uniform = uniform_path(g);
@@ -1363,7 +1368,7 @@ int intp_proxy_create(intp_template t, intp_spline s, int64_t npts, const double
     int* dbad = nullptr;
     hipError_t e = hipSuccess;
     if (bounds) {
-        e = hipMallocAsync(reinterpret_cast<void**>(&dbad), sizeof(int), stream);
+        e = device_alloc(reinterpret_cast<void**>(&dbad), sizeof(int), stream);
         if (e == hipSuccess) e = hipMemsetAsync(dbad, 0, sizeof(int), stream);
         pp.bad = dbad;
     }

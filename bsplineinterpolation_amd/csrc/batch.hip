@@ -639,8 +639,8 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
     if (need > b->solve_cap) {
         release_after(b->device, b->solve_items, b->uses);
         b->solve_items = nullptr;
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->solve_items), need * sizeof(BatchSolveItem), stream),
-                "hipMallocAsync (batch work)");
+        HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->solve_items), need * sizeof(BatchSolveItem), stream),
+                "device allocation (batch work)");
         b->solve_cap = need;
     }
     HIP_TRY(hipMemcpyAsync(b->specs, b->host_specs.data(), b->host_specs.size() * sizeof(BatchSpline),
@@ -684,8 +684,8 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
     if (need > b->eval_cap) {
         release_after(b->device, b->eval_items, b->uses);
         b->eval_items = nullptr;
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->eval_items), need * sizeof(BatchEvalItem), stream),
-                "hipMallocAsync (batch work)");
+        HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->eval_items), need * sizeof(BatchEvalItem), stream),
+                "device allocation (batch work)");
         b->eval_cap = need;
     }
     if (!b->eval_host.empty())
@@ -746,8 +746,8 @@ int prepare_split(intp_batch b, hipStream_t stream) {
         b->host_specs[static_cast<size_t>(s)].nseg = S;
     }
     if (!tab.empty()) {
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->split_tables), tab.size() * sizeof(double), stream),
-                "hipMallocAsync (split lines)");
+        HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->split_tables), tab.size() * sizeof(double), stream),
+                "device allocation (split lines)");
         HIP_TRY(hipMemcpyAsync(b->split_tables, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync");
         for (const auto& [s, off] : at) {
@@ -759,8 +759,8 @@ int prepare_split(intp_batch b, hipStream_t stream) {
         HIP_TRY(hipMemcpyAsync(b->specs, b->host_specs.data(), b->host_specs.size() * sizeof(BatchSpline),
                                hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync");
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->split_list), b->split_host.size() * sizeof(int32_t), stream),
-                "hipMallocAsync (split lines)");
+        HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->split_list), b->split_host.size() * sizeof(int32_t), stream),
+                "device allocation (split lines)");
         HIP_TRY(hipMemcpyAsync(b->split_list, b->split_host.data(), b->split_host.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync");
@@ -1051,14 +1051,14 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
     b->image.assign(std::max<size_t>(image_bytes, 256), 0);
     for (const Item& it : items)
         if (it.bytes) std::memcpy(b->image.data() + it.at, it.src, it.bytes);
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->tables), b->image.size(), stream), "hipMallocAsync (batch)");
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->tables), b->image.size(), stream), "device allocation (batch)");
     HIP_TRY(hipMemcpyAsync(b->tables, b->image.data(), b->image.size(), hipMemcpyHostToDevice, stream),
             "hipMemcpyAsync");
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->specs), static_cast<size_t>(count) * sizeof(BatchSpline), stream),
-            "hipMallocAsync (batch)");
-    HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->ctrl), static_cast<size_t>(std::max<int64_t>(total, 1)) * 8,
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->specs), static_cast<size_t>(count) * sizeof(BatchSpline), stream),
+            "device allocation (batch)");
+    HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->ctrl), static_cast<size_t>(std::max<int64_t>(total, 1)) * 8,
                            stream),
-            "hipMallocAsync (batch control points)");
+            "device allocation (batch control points)");
     HIP_TRY(b->uses.record(stream), "hipEventRecord");
 
     // device descriptors
@@ -1109,8 +1109,8 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
         return b->setups[static_cast<size_t>(x) * kMaxDim].n > b->setups[static_cast<size_t>(y) * kMaxDim].n;
     });
     if (!b->list1d_host.empty()) {
-        HIP_TRY(hipMallocAsync(reinterpret_cast<void**>(&b->list1d), b->list1d_host.size() * sizeof(int32_t), stream),
-                "hipMallocAsync (batch work)");
+        HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->list1d), b->list1d_host.size() * sizeof(int32_t), stream),
+                "device allocation (batch work)");
         HIP_TRY(hipMemcpyAsync(b->list1d, b->list1d_host.data(), b->list1d_host.size() * sizeof(int32_t),
                                hipMemcpyHostToDevice, stream),
                 "hipMemcpyAsync");

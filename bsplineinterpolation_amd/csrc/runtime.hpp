@@ -56,11 +56,21 @@ class DeviceScope {
 
 // ---- device memory lifetimes -------------------------------------------
 //
-// Every device buffer comes from the stream-ordered allocator (the
-// device's default memory pool, kept cached up to kPoolKeepBytes) and is
-// released with hipFreeAsync on a per-device release stream that first
-// waits for every recorded use of the buffer.  Nothing on the build or
-// evaluation path calls hipMalloc/hipFree, which synchronize the device.
+// Every device buffer comes from the library's own stream-ordered memory
+// pool on its device (device_alloc; kept cached up to kPoolKeepBytes) and
+// is released with hipFreeAsync, either on the stream that used it or on
+// a per-device release stream that first waits for every recorded use of
+// the buffer.  Nothing on the build or evaluation path calls
+// hipMalloc/hipFree, which synchronize the device.
+//
+// Reuse policy of the pool: a freed block goes to a later allocation on
+// the stream that freed it (stream order makes that safe), or to another
+// stream once the free has completed on the device.  Reuse through event
+// dependencies is off: with it on, two refined builds on different
+// streams of the serving loop computed into one scratch block; the
+// second was launched after its stream had waited on an event of the
+// first one's stream that was recorded before the first build's free.
+// Reuse through dependencies the pool inserts itself is off as well.
 
 constexpr int kMaxDevices = 64;
 constexpr uint64_t kPoolKeepBytes = uint64_t{16} << 30;
@@ -69,11 +79,34 @@ struct DeviceRuntime {
     bool ready = false;
     hipStream_t release = nullptr;  // frees, ordered after recorded uses
     hipStream_t setup = nullptr;    // uploads of calls that take no stream
+    hipMemPool_t pool = nullptr;    // every device buffer of the library
 };
 
-// Per-device streams and pool settings, created on first use and kept
-// until the process ends.  Both streams are non-blocking with default
-// priority.
+inline hipError_t create_pool(int device, hipMemPool_t* out) {
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    hipMemPool_t pool = nullptr;
+    hipError_t e = hipMemPoolCreate(&pool, &props);
+    if (e != hipSuccess) return e;
+    uint64_t keep = kPoolKeepBytes;
+    int no = 0, yes = 1;
+    e = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolReuseFollowEventDependencies, &no);
+    if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no);
+    if (e == hipSuccess) e = hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowOpportunistic, &yes);
+    if (e != hipSuccess) {
+        (void)hipMemPoolDestroy(pool);
+        return e;
+    }
+    *out = pool;
+    return hipSuccess;
+}
+
+// Per-device streams and memory pool, created on first use and kept until
+// the process ends.  Both streams are non-blocking with default priority.
 inline DeviceRuntime* device_runtime(int device) {
     static std::mutex mu;
     static DeviceRuntime rt[kMaxDevices];
@@ -83,25 +116,23 @@ inline DeviceRuntime* device_runtime(int device) {
     if (!r.ready) {
         DeviceScope scope(device);
         if (!scope.ok()) return nullptr;
-        hipMemPool_t pool;
-        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
-            uint64_t keep = 0;
-            if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess &&
-                keep < kPoolKeepBytes) {
-                keep = kPoolKeepBytes;
-                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-            }
-            // A block freed on one stream is handed to an allocation on
-            // another stream only once the free has completed, never by
-            // dependencies the pool inserts between the two streams.
-            int no = 0;
-            (void)hipMemPoolSetAttribute(pool, hipMemPoolReuseAllowInternalDependencies, &no);
-        }
+        if (!r.pool && create_pool(device, &r.pool) != hipSuccess) return nullptr;
         if (!r.release && hipStreamCreateWithFlags(&r.release, hipStreamNonBlocking) != hipSuccess) return nullptr;
         if (!r.setup && hipStreamCreateWithFlags(&r.setup, hipStreamNonBlocking) != hipSuccess) return nullptr;
         r.ready = true;
     }
     return &r;
+}
+
+// Stream-ordered allocation of `bytes` on the current device from the
+// library's pool, usable on `s` (and, after an event, on other streams).
+inline hipError_t device_alloc(void** p, size_t bytes, hipStream_t s) {
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    DeviceRuntime* rt = device_runtime(device);
+    if (!rt) return hipErrorInvalidDevice;
+    return hipMallocFromPoolAsync(p, bytes, rt->pool, s);
 }
 
 // The streams an object's device memory was used on, each with an event
@@ -159,7 +190,7 @@ class UseTracker {
     std::vector<Use> uses_;
 };
 
-// Frees `p` (from hipMallocAsync) once every use recorded in `uses` is
+// Frees `p` (from device_alloc) once every use recorded in `uses` is
 // done, without blocking the host.
 inline void release_after(int device, void* p, UseTracker& uses) {
     if (!p) return;

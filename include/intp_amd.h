@@ -62,10 +62,13 @@ typedef enum {
                                      read back with intp_eval_timings()   */
 #define INTP_EVAL_OUT_F32 16u     /* `out` points to float storage: results
                                      are stored as FP32 (refined splines) */
-#define INTP_EVAL_GROUP_TWO_PASS 128u /* group with two radix passes over
-                                     small bricks (24 KB patches) even where
-                                     the one-pass grouping over large bricks
-                                     applies                               */
+#define INTP_EVAL_GROUP_ONE_PASS 128u /* group in one counting pass over
+                                     large bricks (64 KB patches, brick items)
+                                     where the grid allows it, instead of two
+                                     radix passes over small bricks: half the
+                                     scratch memory (32 bytes per point), a
+                                     few percent slower at 256^3 (random
+                                     record writes)                        */
 #define INTP_EVAL_HOST_MIRROR 64u /* evaluate on the calling host thread from
                                      a host copy of the control points
                                      (downloaded on the first such call and
@@ -192,10 +195,10 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
  * outside range(d).  Orders above the spline order give 0.
  * Large batches on uniform 2D/3D grids are first grouped by brick of the
  * control-point table so that each workgroup gathers from a patch of the
- * table staged in LDS: one counting pass and one scatter of 32-byte
- * records when the grid has at most 16384 bricks whose patches fit 64 KB,
- * else two radix passes over smaller bricks (scratch device memory from
- * the stream-ordered allocator, about 32 bytes per point plus tables).
+ * table staged in LDS: two radix passes over 32-byte records (scratch
+ * device memory from the stream-ordered allocator, about 68 bytes per
+ * point), or with INTP_EVAL_GROUP_ONE_PASS one counting pass over larger
+ * bricks (about 32 bytes per point).
  * Results are written to the caller's positions and do not depend on the
  * grouping.                                                              */
 int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,

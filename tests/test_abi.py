@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "intp_amd.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(intp_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(intp_\w+)\s*\(", txt, re.M)))
 
 
 @pytest.fixture(scope="module")

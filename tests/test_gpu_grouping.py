@@ -41,9 +41,9 @@ def _eval_both(g, q, orders, method=True):
     return a, out.cpu().numpy()
 
 
-# method True: one pass over large bricks (brick items) where the grid
-# allows it; "two_pass": two radix passes over small bricks
-@pytest.mark.parametrize("method", [True, "two_pass"])
+# method True: two radix passes over small bricks (the default);
+# "one_pass": one counting pass over large bricks (brick items)
+@pytest.mark.parametrize("method", [True, "one_pass"])
 @pytest.mark.parametrize("shape,k,per,npts", [
     ((96, 80, 64), 3, [True, False, False], (1 << 21) + 3),
     ((40, 33, 50), 5, [False, True, True], (1 << 20)),
@@ -112,7 +112,7 @@ def test_gpu_grouped_skewed_points(amd):
     q[:, hot:] = rng.uniform(-0.3, 1.3, (3, n - hot))
     q[0, hot: hot + 100] += 7.0
     orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
-    for method in (True, "two_pass"):
+    for method in (True, "one_pass"):
         grouped, plain = _eval_both(g, q, orders, method)
         assert np.array_equal(grouped, plain), method
 
@@ -130,5 +130,5 @@ def test_gpu_forced_grouping_of_small_batches(amd, npts):
     orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
     a = g.eval(q, orders, group=True).cpu().numpy()
     b = g.eval(q, orders, group=False).cpu().numpy()
-    c = g.eval(q, orders, group="two_pass").cpu().numpy()
+    c = g.eval(q, orders, group="one_pass").cpu().numpy()
     assert np.array_equal(a, b) and np.array_equal(c, b)
