@@ -56,6 +56,7 @@ EXPORTS = (
     "intp_server_create", "intp_server_destroy", "intp_serve_build", "intp_serve_eval", "intp_serve_poll",
     "intp_serve_wait", "intp_serve_drain", "intp_serve_release", "intp_serve_spline", "intp_serve_stats",
     "intp_proxy_create", "intp_proxy_eval", "intp_proxy_points", "intp_proxy_destroy",
+    "intp_eval_static", "intp_eval_static_capacity",
 )
 
 
@@ -153,6 +154,9 @@ def load(path: str = LIB_PATH):
     lib.intp_proxy_eval.argtypes = [vp, vp, dp, C.c_int64, C.c_uint32, vp]
     lib.intp_proxy_points.argtypes = [vp, i64p]
     lib.intp_proxy_destroy.argtypes = [vp]
+    lib.intp_eval_static.argtypes = lib.intp_eval.argtypes
+    lib.intp_eval_static_capacity.argtypes = []
+    lib.intp_eval_static_capacity.restype = C.c_int64
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype or C.c_int
     lib.intp_last_error.restype = C.c_char_p

@@ -590,6 +590,41 @@ bool uniform_path(const GridData& g) {
 
 }  // namespace
 
+// ---- the fixed scratch buffer of intp_eval_static ------------------------
+//
+// One file-scope device array; every GPU the library runs on holds its own
+// copy (kStaticScratchBytes) until the process ends.  Host code never
+// passes the array's name to a kernel: its device address on each GPU is
+// resolved once per device with hipGetSymbolAddress (when a context is
+// created on that device) and kept in g_static_scratch.
+constexpr size_t kStaticScratchBytes = size_t{128} << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+std::mutex g_static_mu;
+unsigned char* g_static_scratch[kMaxDevices] = {};
+
+int resolve_static_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(INTP_E_INVALID, "device index beyond the supported range");
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (g_static_scratch[device]) return INTP_OK;
+    DeviceScope scope(device);
+    if (!scope.ok()) return fail(INTP_E_HIP, "cannot select device");
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress (static scratch)");
+    g_static_scratch[device] = static_cast<unsigned char*>(ptr);
+    return INTP_OK;
+}
+
+unsigned char* static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    return device >= 0 && device < kMaxDevices ? g_static_scratch[device] : nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* intp_last_error(void) { return g_error.c_str(); }
@@ -613,6 +648,8 @@ int intp_ctx_create(int device, intp_ctx* out) {
     HIP_TRY(hipGetDeviceCount(&count), "hipGetDeviceCount");
     if (device < 0 || device >= count) return fail(INTP_E_INVALID, "no such device");
     if (!device_runtime(device)) return fail(INTP_E_HIP, "cannot create the device's streams");
+    const int rc = resolve_static_scratch(device);
+    if (rc != INTP_OK) return rc;
     *out = new intp_ctx_s{device};
     return INTP_OK;
 }
@@ -882,7 +919,7 @@ int64_t first_outside_host(const GridData& g, int64_t npts, const double* coords
 
 // Device scratch of one evaluation call (bounds flag, grouping records
 // and tables): from the stream-ordered pool, or carved out of a fixed
-// buffer that the caller owns for the call.
+// buffer that the caller owns for the call (intp_eval_static).
 struct Scratch {
     hipStream_t stream = nullptr;
     unsigned char* base = nullptr;  // fixed buffer, or null: the pool
@@ -918,6 +955,22 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     return eval_call(s, npts, coords, cs_dim, cs_pt, nout, orders, out, os_out, os_pt, flags, pool.stream, pool);
 }
 
+int intp_eval_static(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt, int32_t nout,
+                     const int32_t* orders, double* out, int64_t os_out, int64_t os_pt, uint32_t flags,
+                     void* stream_) {
+    if (!s) return fail(INTP_E_INVALID, "null spline");
+    if (flags & (INTP_EVAL_HOST_MEM | INTP_EVAL_HOST_MIRROR | INTP_EVAL_TIMED))
+        return fail(INTP_E_INVALID, "intp_eval_static takes device buffers only (no HOST_MEM, HOST_MIRROR or TIMED)");
+    unsigned char* base = static_scratch(s->grid->device);
+    if (!base) return fail(INTP_E_INVALID, "no context was created on the spline's device");
+    Scratch fixed;
+    fixed.stream = static_cast<hipStream_t>(stream_);
+    fixed.base = base;
+    fixed.cap = kStaticScratchBytes;
+    return eval_call(s, npts, coords, cs_dim, cs_pt, nout, orders, out, os_out, os_pt, flags, fixed.stream, fixed);
+}
+
+int64_t intp_eval_static_capacity(void) { return static_cast<int64_t>(kStaticScratchBytes); }
 
 }  // extern "C"
 

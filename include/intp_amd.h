@@ -205,6 +205,26 @@ int intp_eval(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
               int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
               uint32_t flags, void* stream);
 
+/* intp_eval without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, kernels and results as intp_eval, but every scratch buffer
+ * (the bounds flag, the grouping's records and tables) is carved from one
+ * fixed device buffer per GPU of intp_eval_static_capacity() bytes
+ * (128 MiB), held from library load until the process ends; the call
+ * makes no hipMalloc/hipFree or other allocation.  Device buffers only:
+ * INTP_EVAL_HOST_MEM, INTP_EVAL_HOST_MIRROR and INTP_EVAL_TIMED are
+ * refused.  A batch whose grouping would not fit the buffer is evaluated
+ * in caller order (same results); with INTP_EVAL_GROUPED it is refused
+ * (INTP_E_INVALID).  The buffer is shared by every call on that GPU and
+ * keeps the last call's contents: calls that overlap on one GPU (other
+ * threads, other streams) race and must be serialized by the caller.  A
+ * context must have been created on the spline's device.                */
+int intp_eval_static(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
+                     int32_t nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt,
+                     uint32_t flags, void* stream);
+int64_t intp_eval_static_capacity(void);
+
 /* With INTP_EVAL_BOUNDS_CHECK, INTP_E_DOMAIN names the first offending
  * point (index, dimension, coordinate and range) in intp_last_error();
  * the outputs of a batch that fails the check are unspecified.           */
