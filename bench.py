@@ -518,23 +518,15 @@ def cpu_baseline(w, gpu_out_sample, sample_mt: int, sample_st: int):
 
 def measure_memory_probes(torch, n: int = 1 << 26, reps: int = 5):
     """Reference rates for the eval kernel's memory pattern: 32-byte rows
-    (value + 3 partials per point) copied in order, scattered to a random
-    permutation (the grouped kernel's result stores) and gathered from
-    one (the un-permute a sorted-order store would need)."""
+    (value + 3 partials per point) copied in order and scattered to a
+    random permutation (the grouped kernel's result stores)."""
     dev = torch.device("cuda:0")
     g = torch.Generator(device="cpu").manual_seed(3)
     src = torch.rand((n, 4), generator=g, dtype=torch.float64).to(dev)
     dst = torch.empty_like(src)
     perm = torch.randperm(n, generator=g).to(dev)
-    chunk = 1 << 22  # index_select launches over 2^24 elements at most
-
-    def gather():
-        for i in range(0, n, chunk):
-            torch.index_select(src, 0, perm[i:i + chunk], out=dst[i:i + chunk])
-
     probes = (("sequential_rows", lambda: dst.copy_(src)),
-              ("random_row_scatter", lambda: dst.index_copy_(0, perm, src)),
-              ("random_row_gather", gather))
+              ("random_row_scatter", lambda: dst.index_copy_(0, perm, src)))
     res = {}
     for name, fn in probes:
         for _ in range(2):
@@ -570,7 +562,7 @@ def main():
     ap.add_argument("--only-batched", action="store_true",
                     help="measure only the batched form (4096 mixed splines); prints its object")
     ap.add_argument("--memory-probes", action="store_true",
-                    help="time 32-byte row copies in order, scattered and gathered (2^26 rows); prints their object")
+                    help="time 32-byte row copies in order and scattered (2^26 rows); prints their object")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="copy 256 MiB once before timing (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
