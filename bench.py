@@ -544,6 +544,24 @@ def measure_memory_probes(torch, n: int = 1 << 26, reps: int = 5):
     return dict(rows=n, row_bytes=32, probes=res)
 
 
+def measure_drop_in_timing():
+    """Per-call cost of the C++ drop-in's scalar members and eval_proxy
+    objects (tests/cpp/drop_in_timing.cpp), with the compiled reference's
+    operator() timed on the same meshes and points when oracle/_ref is
+    built (the CPU comparison of this leg)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "drop_in_timing")
+    if not os.path.exists(exe):
+        raise SystemExit("bench.py: %s is not built (python -c 'import __graft_entry__ as g; g.build()')" % exe)
+    ref = os.path.join(ROOT, "oracle", "_ref", "libintp_ref.so")
+    cmd = [exe] + ([ref] if os.path.exists(ref) else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, check=True)
+    rows = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    return dict(workload="2^15 random points sorted by cell, 32 passes, meshes of 2^18 points (1D 262144, "
+                         "2D 512^2, 3D 64^3), even dimensions periodic; reference operator() on one thread",
+                rows=rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -561,6 +579,8 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=1500, help="requests of the saturation burst")
     ap.add_argument("--only-batched", action="store_true",
                     help="measure only the batched form (4096 mixed splines); prints its object")
+    ap.add_argument("--drop-in-timing", action="store_true",
+                    help="per-call cost of the C++ drop-in's scalar members and proxies; prints their object")
     ap.add_argument("--memory-probes", action="store_true",
                     help="time 32-byte row copies in order and scattered (2^26 rows); prints their object")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -570,6 +590,9 @@ def main():
         print("bench.py: runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.drop_in_timing:
+        print(json.dumps(measure_drop_in_timing()))
+        return
     import torch
     import bsplineinterpolation_amd as amd
     from bsplineinterpolation_amd import datagen

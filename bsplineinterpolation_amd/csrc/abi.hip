@@ -180,6 +180,7 @@ struct intp_spline_s {
     // Host copy of the control points (FP64), downloaded on the first
     // host-side evaluation; immutable afterwards like the device table.
     std::once_flag mirror_once;
+    std::atomic<bool> mirror_ready{false};  // set once `mirror` holds the table
     int mirror_rc = INTP_OK;
     std::string mirror_error;
     std::vector<double> mirror;
@@ -869,13 +870,16 @@ namespace {
 
 // The spline's control points on the host (FP64), downloaded once.
 int ensure_mirror(intp_spline s) {
+    if (s->mirror_ready.load(std::memory_order_acquire)) return INTP_OK;  // the per-call path
     std::call_once(s->mirror_once, [s]() {
         std::vector<double> host(static_cast<size_t>(s->grid->total));
         s->mirror_rc = intp_spline_download_ctrl(s, host.data(), s->grid->total);
-        if (s->mirror_rc == INTP_OK)
+        if (s->mirror_rc == INTP_OK) {
             s->mirror = std::move(host);
-        else
+            s->mirror_ready.store(true, std::memory_order_release);
+        } else {
             s->mirror_error = g_error;
+        }
     });
     if (s->mirror_rc != INTP_OK) return fail(s->mirror_rc, s->mirror_error);
     return INTP_OK;

@@ -10,18 +10,23 @@
 //    derivatives (BSpline.hpp:83-111);
 //  * derivatives by differencing the gathered (order + 1)^D control points
 //    along each differentiated dimension (BSpline.hpp:489-520), then one
-//    dot product with the tensor-product basis.
+//    dot product with the tensor-product basis;
+//  * the products and sums in the reference's own order, so that values,
+//    derivatives and proxy results equal the reference's bit for bit.
+// Dimension and order are template arguments (a table of instantiations):
+// the stencil loops unroll and no index is split by division.
 #include "host_eval.hpp"
 
+#include <array>
 #include <cmath>
 #include <cstddef>
+#include <utility>
 
 namespace intp_amd {
 
 namespace {
 
 constexpr int kMaxK = 7;
-constexpr int kMaxStencil = (kMaxK + 1) * (kMaxK + 1) * (kMaxK + 1);
 
 // Wraps a periodic coordinate and returns its knot span.
 int64_t span_of(const DimSetup& s, int k, double& x) {
@@ -61,67 +66,206 @@ inline int64_t stencil_index(const DimSetup& s, int64_t first, int m) {
     return id;
 }
 
-double eval_point(const HostSpline& s, const double* xin, const int32_t* r) {
-    const int D = s.dim, k = s.order, w = k + 1;
-    for (int d = 0; d < D; ++d)
-        if (r[d] > k) return 0.0;
-    double x[3];
-    int64_t mu[3], stride[3];
-    double basis[3][kMaxK + 1];
-    int64_t st = 1;
+// Per dimension: wrap, span, the K + 1 basis values of order K - r
+// (right-aligned) and the table offsets of the stencil's elements.
+template <int K>
+inline int64_t dimension_terms(const DimSetup& ds, int64_t stride, double x, int r, double* b, int64_t* off) {
+    const int64_t mu = span_of(ds, K, x);
+    basis_values(ds.knots.data(), mu, x, K, K - r, b);
+    const int64_t first = mu - K;
+    for (int m = 0; m <= K; ++m) off[m] = stencil_index(ds, first, m) * stride;
+    return mu;
+}
+
+// All dimensions; returns the spans in mu.
+template <int D, int K>
+inline void all_terms(const HostSpline& s, const double* x, const int32_t* r, double (&b)[3][K + 1],
+                      int64_t (&off)[3][K + 1], int64_t (&mu)[3]) {
+    int64_t stride = 1;
     for (int d = D - 1; d >= 0; --d) {
-        stride[d] = st;
-        st *= s.dims[d]->n;
+        mu[d] = dimension_terms<K>(*s.dims[d], stride, x[d], r ? r[d] : 0, b[d], off[d]);
+        stride *= s.dims[d]->n;
     }
-    for (int d = 0; d < D; ++d) {
-        x[d] = xin[d];
-        mu[d] = span_of(*s.dims[d], k, x[d]);
-        basis_values(s.dims[d]->knots.data(), mu[d], x[d], k, k - r[d], basis[d]);
-    }
-    // gather the local (k+1)^D control points, C order
-    double c[kMaxStencil];
-    int count = 1;
-    for (int d = 0; d < D; ++d) count *= w;
-    for (int l = 0; l < count; ++l) {
-        int64_t at = 0;
-        int rem = l;
-        for (int d = D - 1; d >= 0; --d) {
-            const int m = rem % w;
-            rem /= w;
-            at += stencil_index(*s.dims[d], mu[d] - k, m) * stride[d];
+}
+
+// Value, summed as the reference's operator() does with its cell layout
+// (BSpline.hpp:305-335): stencil elements with dimension 0 varying
+// fastest, each term ((c * b_0) * b_1) * b_2, added in that order.
+template <int D, int K>
+double value_at(const HostSpline& s, const double* x) {
+    constexpr int W = K + 1;
+    double b[3][W];
+    int64_t off[3][W], mu[3];
+    all_terms<D, K>(s, x, nullptr, b, off, mu);
+    const double* c = s.ctrl;
+    double v = 0.0;
+    if constexpr (D == 1) {
+        for (int i0 = 0; i0 < W; ++i0) {
+            double coef = c[off[0][i0]];
+            coef *= b[0][i0];
+            v += coef;
         }
-        c[l] = s.ctrl[at];
+    } else if constexpr (D == 2) {
+        for (int i1 = 0; i1 < W; ++i1)
+            for (int i0 = 0; i0 < W; ++i0) {
+                double coef = c[off[0][i0] + off[1][i1]];
+                coef *= b[0][i0];
+                coef *= b[1][i1];
+                v += coef;
+            }
+    } else {
+        for (int i2 = 0; i2 < W; ++i2)
+            for (int i1 = 0; i1 < W; ++i1)
+                for (int i0 = 0; i0 < W; ++i0) {
+                    double coef = c[off[0][i0] + off[1][i1] + off[2][i2]];
+                    coef *= b[0][i0];
+                    coef *= b[1][i1];
+                    coef *= b[2][i2];
+                    v += coef;
+                }
     }
-    // difference along each differentiated dimension
+    return v;
+}
+
+// Partial derivative, as the reference's derivative_at (BSpline.hpp:393-531):
+// the local (K+1)^D control points in C order, differenced along each
+// differentiated dimension, then sum over C order of (b_0 b_1 b_2) * c.
+template <int D, int K>
+double derivative_at(const HostSpline& s, const double* x, const int32_t* r) {
+    constexpr int W = K + 1;
+    constexpr int N = D == 1 ? W : D == 2 ? W * W : W * W * W;
+    for (int d = 0; d < D; ++d)
+        if (r[d] > K) return 0.0;
+    double b[3][W];
+    int64_t off[3][W], mu[3];
+    all_terms<D, K>(s, x, r, b, off, mu);
+    double c[N];
+    int l = 0;
+    if constexpr (D == 1) {
+        for (int i0 = 0; i0 < W; ++i0) c[l++] = s.ctrl[off[0][i0]];
+    } else if constexpr (D == 2) {
+        for (int i0 = 0; i0 < W; ++i0)
+            for (int i1 = 0; i1 < W; ++i1) c[l++] = s.ctrl[off[0][i0] + off[1][i1]];
+    } else {
+        for (int i0 = 0; i0 < W; ++i0)
+            for (int i1 = 0; i1 < W; ++i1)
+                for (int i2 = 0; i2 < W; ++i2) c[l++] = s.ctrl[off[0][i0] + off[1][i1] + off[2][i2]];
+    }
     for (int d = 0; d < D; ++d) {
         if (r[d] == 0) continue;
         const double* t = s.dims[d]->knots.data();
         int lstride = 1;
-        for (int e = d + 1; e < D; ++e) lstride *= w;
-        const int fibers = count / w;
+        for (int e = d + 1; e < D; ++e) lstride *= W;
+        constexpr int fibers = N / W;
         for (int f = 0; f < fibers; ++f) {
             const int outer = f / lstride, inner = f % lstride;
-            double* v = c + outer * lstride * w + inner;
-            for (int kk = k; kk > k - r[d]; --kk)
+            double* v = c + outer * lstride * W + inner;
+            for (int kk = K; kk > K - r[d]; --kk)
                 for (int j = kk; j > 0; --j) {
-                    const int at = k + j - kk;
+                    const int at = K + j - kk;
                     v[at * lstride] = static_cast<double>(kk) * (v[at * lstride] - v[(at - 1) * lstride]) /
                                       (t[mu[d] + j] - t[mu[d] + j - kk]);
                 }
         }
     }
-    double val = 0.0;
-    for (int l = 0; l < count; ++l) {
-        double coef = 1.0;
-        int rem = l;
-        for (int d = D - 1; d >= 0; --d) {
-            coef *= basis[d][rem % w];
-            rem /= w;
-        }
-        val += coef * c[l];
+    double v = 0.0;
+    l = 0;
+    if constexpr (D == 1) {
+        for (int i0 = 0; i0 < W; ++i0) v += b[0][i0] * c[l++];
+    } else if constexpr (D == 2) {
+        for (int i0 = 0; i0 < W; ++i0)
+            for (int i1 = 0; i1 < W; ++i1) {
+                double coef = b[0][i0];
+                coef *= b[1][i1];
+                v += coef * c[l++];
+            }
+    } else {
+        for (int i0 = 0; i0 < W; ++i0)
+            for (int i1 = 0; i1 < W; ++i1)
+                for (int i2 = 0; i2 < W; ++i2) {
+                    double coef = b[0][i0];
+                    coef *= b[1][i1];
+                    coef *= b[2][i2];
+                    v += coef * c[l++];
+                }
     }
-    return val;
+    return v;
 }
+
+template <int D, int K>
+double eval_point_t(const HostSpline& s, const double* x, const int32_t* r) {
+    for (int d = 0; d < D; ++d)
+        if (r[d] != 0) return derivative_at<D, K>(s, x, r);
+    return value_at<D, K>(s, x);
+}
+
+using PointFn = double (*)(const HostSpline&, const double*, const int32_t*);
+
+template <int D, int... K>
+constexpr std::array<PointFn, sizeof...(K)> point_fns(std::integer_sequence<int, K...>) {
+    return {&eval_point_t<D, K>...};
+}
+
+const std::array<std::array<PointFn, kMaxK + 1>, 3> kPointFns = {
+    point_fns<1>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    point_fns<2>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    point_fns<3>(std::make_integer_sequence<int, kMaxK + 1>{})};
+
+// Proxy application, summed like value_at (the reference's operator()
+// order), so that a host proxy gives exactly the value the scalar call
+// gives.  (The reference's own pre_calc_coef closure, BSpline.hpp:270-292,
+// multiplies the weights first and may differ from its operator() in the
+// last bit.)
+template <int D, int K>
+double apply_weights(const HostSpline& s, const int32_t* first, const double* w) {
+    constexpr int W = K + 1;
+    int64_t off[3][W];
+    int64_t stride = 1;
+    for (int d = D - 1; d >= 0; --d) {
+        for (int m = 0; m < W; ++m) off[d][m] = stencil_index(*s.dims[d], first[d], m) * stride;
+        stride *= s.dims[d]->n;
+    }
+    const double* c = s.ctrl;
+    double v = 0.0;
+    if constexpr (D == 1) {
+        for (int i0 = 0; i0 < W; ++i0) {
+            double coef = c[off[0][i0]];
+            coef *= w[i0];
+            v += coef;
+        }
+    } else if constexpr (D == 2) {
+        for (int i1 = 0; i1 < W; ++i1)
+            for (int i0 = 0; i0 < W; ++i0) {
+                double coef = c[off[0][i0] + off[1][i1]];
+                coef *= w[i0];
+                coef *= w[W + i1];
+                v += coef;
+            }
+    } else {
+        for (int i2 = 0; i2 < W; ++i2)
+            for (int i1 = 0; i1 < W; ++i1)
+                for (int i0 = 0; i0 < W; ++i0) {
+                    double coef = c[off[0][i0] + off[1][i1] + off[2][i2]];
+                    coef *= w[i0];
+                    coef *= w[W + i1];
+                    coef *= w[2 * W + i2];
+                    v += coef;
+                }
+    }
+    return v;
+}
+
+using ApplyFn = double (*)(const HostSpline&, const int32_t*, const double*);
+
+template <int D, int... K>
+constexpr std::array<ApplyFn, sizeof...(K)> apply_fns(std::integer_sequence<int, K...>) {
+    return {&apply_weights<D, K>...};
+}
+
+const std::array<std::array<ApplyFn, kMaxK + 1>, 3> kApplyFns = {
+    apply_fns<1>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    apply_fns<2>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    apply_fns<3>(std::make_integer_sequence<int, kMaxK + 1>{})};
 
 }  // namespace
 
@@ -134,7 +278,8 @@ int64_t host_eval_points(const HostSpline& s, int64_t npts, const double* coords
         if (bounds)
             for (int d = 0; d < D; ++d)
                 if (outside(*s.dims[d], x[d])) return i;
-        for (int o = 0; o < nout; ++o) out[o * os_out + i * os_pt] = eval_point(s, x, orders + o * D);
+        const PointFn fn = kPointFns[D - 1][s.order];
+        for (int o = 0; o < nout; ++o) out[o * os_out + i * os_pt] = fn(s, x, orders + o * D);
     }
     return -1;
 }
@@ -157,30 +302,8 @@ int64_t host_proxy_weights(const HostSpline& s, int64_t npts, const double* coor
 void host_proxy_apply(const HostSpline& s, int64_t npts, const int32_t* first, const double* weights, double* out,
                       int64_t os_pt) {
     const int D = s.dim, w = s.order + 1;
-    int64_t stride[3];
-    int64_t st = 1;
-    for (int d = D - 1; d >= 0; --d) {
-        stride[d] = st;
-        st *= s.dims[d]->n;
-    }
-    int count = 1;
-    for (int d = 0; d < D; ++d) count *= w;
-    for (int64_t i = 0; i < npts; ++i) {
-        double v = 0.0;
-        for (int l = 0; l < count; ++l) {
-            int64_t at = 0;
-            double coef = 1.0;
-            int rem = l;
-            for (int d = D - 1; d >= 0; --d) {
-                const int m = rem % w;
-                rem /= w;
-                at += stencil_index(*s.dims[d], first[i * D + d], m) * stride[d];
-                coef *= weights[(i * D + d) * w + m];
-            }
-            v += coef * s.ctrl[at];
-        }
-        out[i * os_pt] = v;
-    }
+    const ApplyFn fn = kApplyFns[D - 1][s.order];
+    for (int64_t i = 0; i < npts; ++i) out[i * os_pt] = fn(s, first + i * D, weights + i * D * w);
 }
 
 }  // namespace intp_amd

@@ -129,5 +129,17 @@ def build(out_dir: str = os.path.join(HERE, "_build")) -> str:
     return exe
 
 
+def build_timing(out_dir: str = os.path.join(HERE, "_build")) -> str:
+    """The scalar-call timing program (tests/cpp/drop_in_timing.cpp)."""
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "drop_in_timing")
+    lib_dir = os.path.join(ROOT, "bsplineinterpolation_amd")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(HERE, "drop_in_timing.cpp"), "-o", exe,
+           "-L" + lib_dir, "-lintp_amd", "-ldl", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib_dir, out_dir)]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "_build")))

@@ -60,6 +60,7 @@ def test_gpu_eval_proxy_matches_oracle(amd, orc, D, k, p):
     got_h = hp(g)
     assert rel_err(got_h, want) < TOL, rel_err(got_h, want)
     assert np.array_equal(got_h, g.eval_host(q)[0])
+    assert np.array_equal(got_h, want)  # the oracle's (the reference's operator()) bits
     # device proxy
     qd = torch.from_numpy(q).cuda()
     dp = t.eval_proxy(qd)
@@ -128,8 +129,9 @@ HOST_CONFIGS = [(D, k, per, nu) for D in (1, 2, 3) for k in range(0, 8)
                                                                        "_nu" if c[3] else "") for c in HOST_CONFIGS])
 def test_gpu_host_eval_matches_oracle(amd, orc, cfg):
     """The scalar path: values and derivatives (first, mixed, above the
-    order) from the host copy, against the oracle; extrapolation and exact
-    knot hits included."""
+    order) from the host copy, against the oracle, bit for bit (the host
+    path sums in the reference's order); extrapolation and exact knot hits
+    included."""
     D, k, per, nu = cfg
     rng = np.random.default_rng(hash((D, k, tuple(per), nu, 7)) & 0xFFFFFFFF)
     f, ranges, coords = _rand_cfg(rng, D, k, per, nu)
@@ -149,6 +151,8 @@ def test_gpu_host_eval_matches_oracle(amd, orc, cfg):
             assert np.all(got[i] == 0), r
         else:
             assert rel_err(got[i], want[i]) < TOL, (r, rel_err(got[i], want[i]))
+        # the host path sums in the reference's order: the oracle's bits
+        assert np.array_equal(got[i], want[i]), (r, int(np.sum(got[i] != want[i])))
 
 
 def test_gpu_host_eval_bounds_and_refined(amd, orc):
