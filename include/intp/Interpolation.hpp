@@ -183,11 +183,33 @@ class MeshDimension {
     }
     size_type dim_size(size_type d) const { return dim_size_[d]; }
     size_type& dim_size(size_type d) { return dim_size_[d]; }
+    // product of the sizes of the last dim_ind dimensions (Mesh.hpp:45-51)
+    size_type dim_acc_size(size_type dim_ind) const {
+        size_type s = 1;
+        for (size_type d = 0; d < dim_ind; ++d) s *= dim_size_[D - d - 1];
+        return s;
+    }
     // flat C-order index (last index fastest), Mesh.hpp:87-95
     size_type indexing(const index_type& ind) const {
         size_type flat = 0;
         for (size_type d = 0; d < D; ++d) flat = flat * dim_size_[d] + ind[d];
         return flat;
+    }
+    // indexing with the reference's range check, active like the
+    // reference's INTP_ASSERT (Mesh.hpp:64-85, util.hpp:247-258)
+    template <typename... Indices>
+    size_type indexing_safe(Indices... ind) const {
+        return indexing_safe(index_type{static_cast<size_type>(ind)...});
+    }
+    size_type indexing_safe(const index_type& ind) const {
+#if defined(INTP_ENABLE_ASSERTION) || defined(INTP_DEBUG)
+        for (size_type d = D; d-- > 0;)
+            if (ind[d] >= dim_size_[d])
+                throw std::runtime_error("Mesh access out of range at dim " + std::to_string(d) + ", index " +
+                                         std::to_string(ind[d]) + " is out of [0, " +
+                                         std::to_string(dim_size_[d] - 1) + "].");
+#endif
+        return indexing(ind);
     }
     index_type dimwise_indices(size_type flat) const {
         index_type ind{};
@@ -205,39 +227,111 @@ class MeshDimension {
 };
 
 // C-order value mesh on the host (Mesh.hpp:126-383).
-template <typename T, std::size_t D>
+template <typename T, std::size_t D, typename Alloc = std::allocator<T>>
 class Mesh {
    public:
     using size_type = std::size_t;
     using val_type = T;
+    using allocator_type = Alloc;
     using index_type = typename MeshDimension<D>::index_type;
-    using iterator = typename std::vector<T>::iterator;
-    using const_iterator = typename std::vector<T>::const_iterator;
+    using iterator = typename std::vector<T, Alloc>::iterator;
+    using const_iterator = typename std::vector<T, Alloc>::const_iterator;
     constexpr static size_type dim = D;
 
-    explicit Mesh(MeshDimension<D> dimension) : dimension_(dimension), storage_(dimension.size()) {}
+    // Iterator along one dimension of the mesh (a fixed stride through the
+    // storage), Mesh.hpp:138-236.
+    template <typename V>
+    class skip_iterator {
+       public:
+        using value_type = V;
+        using difference_type = std::ptrdiff_t;
+        using pointer = V*;
+        using reference = V&;
+        using iterator_category = std::random_access_iterator_tag;
+
+        skip_iterator(pointer ptr, difference_type step) : ptr_(ptr), step_(step) {}
+        operator skip_iterator<const V>() const { return {ptr_, step_}; }
+        explicit operator const V*() const { return ptr_; }
+
+        reference operator*() const { return *ptr_; }
+        pointer operator->() const { return ptr_; }
+        reference operator[](difference_type n) const { return *(ptr_ + n * step_); }
+        bool operator==(const skip_iterator& o) const { return ptr_ == o.ptr_ && step_ == o.step_; }
+        bool operator!=(const skip_iterator& o) const { return !(*this == o); }
+        skip_iterator& operator++() {
+            ptr_ += step_;
+            return *this;
+        }
+        skip_iterator operator++(int) {
+            skip_iterator t(*this);
+            ++*this;
+            return t;
+        }
+        skip_iterator& operator--() {
+            ptr_ -= step_;
+            return *this;
+        }
+        skip_iterator operator--(int) {
+            skip_iterator t(*this);
+            --*this;
+            return t;
+        }
+        skip_iterator& operator+=(difference_type n) {
+            ptr_ += n * step_;
+            return *this;
+        }
+        skip_iterator& operator-=(difference_type n) {
+            ptr_ -= n * step_;
+            return *this;
+        }
+        skip_iterator operator+(difference_type n) const { return skip_iterator(*this) += n; }
+        friend skip_iterator operator+(difference_type n, skip_iterator it) { return it += n; }
+        skip_iterator operator-(difference_type n) const { return skip_iterator(*this) -= n; }
+        difference_type operator-(const skip_iterator& o) const { return (ptr_ - o.ptr_) / step_; }
+        bool operator<(const skip_iterator& o) const { return o - *this > 0; }
+        bool operator>(const skip_iterator& o) const { return o < *this; }
+        bool operator<=(const skip_iterator& o) const { return !(*this > o); }
+        bool operator>=(const skip_iterator& o) const { return !(*this < o); }
+
+       private:
+        pointer ptr_;
+        difference_type step_;
+    };
+
+    explicit Mesh(MeshDimension<D> dimension, const Alloc& alloc = Alloc())
+        : dimension_(dimension), storage_(dimension.size(), T{}, alloc) {}
     explicit Mesh(index_type sizes) : Mesh(MeshDimension<D>(sizes)) {}
     template <typename... Args, typename = typename std::enable_if<(sizeof...(Args) == D && D > 1)>::type>
     explicit Mesh(Args... sizes) : Mesh(MeshDimension<D>(index_type{static_cast<size_type>(sizes)...})) {}
-    explicit Mesh(size_type n) : Mesh(MeshDimension<D>(n)) {}
+    explicit Mesh(size_type n, const Alloc& alloc = Alloc()) : Mesh(MeshDimension<D>(n), alloc) {}
     // 1D mesh from an iterator range
     template <typename It, typename = typename std::enable_if<D == 1 && detail::is_iterator<It>::value>::type>
-    explicit Mesh(std::pair<It, It> range) : dimension_(index_type{0}) {
+    explicit Mesh(std::pair<It, It> range, const Alloc& alloc = Alloc())
+        : dimension_(index_type{0}), storage_(alloc) {
         for (It it = range.first; it != range.second; ++it) storage_.push_back(static_cast<T>(*it));
         dimension_.resize(index_type{storage_.size()});
+    }
+    // a mesh with another allocator (Mesh.hpp:276-283)
+    template <typename A2>
+    Mesh(const Mesh<T, D, A2>& other, const Alloc& alloc = Alloc()) : Mesh(other.dimension(), alloc) {
+        std::copy(other.begin(), other.end(), storage_.begin());
     }
 
     size_type size() const { return storage_.size(); }
     size_type dim_size(size_type d) const { return dimension_.dim_size(d); }
     const MeshDimension<D>& dimension() const { return dimension_; }
+    void resize(index_type sizes) {
+        dimension_.resize(sizes);
+        storage_.resize(dimension_.size());
+    }
 
     template <typename... Indices>
     T& operator()(Indices... ind) {
-        return storage_[dimension_.indexing(index_type{static_cast<size_type>(ind)...})];
+        return storage_[dimension_.indexing_safe(ind...)];
     }
     template <typename... Indices>
     const T& operator()(Indices... ind) const {
-        return storage_[dimension_.indexing(index_type{static_cast<size_type>(ind)...})];
+        return storage_[dimension_.indexing_safe(ind...)];
     }
     T& operator()(index_type ind) { return storage_[dimension_.indexing(ind)]; }
     const T& operator()(index_type ind) const { return storage_[dimension_.indexing(ind)]; }
@@ -248,13 +342,37 @@ class Mesh {
     iterator end() { return storage_.end(); }
     const_iterator begin() const { return storage_.cbegin(); }
     const_iterator end() const { return storage_.cend(); }
+    // along dimension dim_ind through `indices` (its own entry ignored)
+    skip_iterator<T> begin(size_type dim_ind, index_type indices) {
+        indices[dim_ind] = 0;
+        return {storage_.data() + dimension_.indexing_safe(indices), stride(dim_ind)};
+    }
+    skip_iterator<T> end(size_type dim_ind, index_type indices) {
+        indices[dim_ind] = dimension_.dim_size(dim_ind);
+        return {storage_.data() + dimension_.indexing(indices), stride(dim_ind)};
+    }
+    skip_iterator<const T> begin(size_type dim_ind, index_type indices) const {
+        indices[dim_ind] = 0;
+        return {storage_.data() + dimension_.indexing_safe(indices), stride(dim_ind)};
+    }
+    skip_iterator<const T> end(size_type dim_ind, index_type indices) const {
+        indices[dim_ind] = dimension_.dim_size(dim_ind);
+        return {storage_.data() + dimension_.indexing(indices), stride(dim_ind)};
+    }
     index_type iter_indices(const_iterator it) const {
         return dimension_.dimwise_indices(static_cast<size_type>(it - storage_.cbegin()));
     }
+    index_type iter_indices(skip_iterator<const T> it) const {
+        return dimension_.dimwise_indices(static_cast<size_type>(static_cast<const T*>(it) - storage_.data()));
+    }
 
    private:
+    std::ptrdiff_t stride(size_type dim_ind) const {
+        return static_cast<std::ptrdiff_t>(dimension_.dim_acc_size(D - dim_ind - 1));
+    }
+
     MeshDimension<D> dimension_;
-    std::vector<T> storage_;
+    std::vector<T, Alloc> storage_;
 };
 
 template <typename T, std::size_t D, std::size_t O, typename U = double>
