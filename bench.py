@@ -157,8 +157,10 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
     b = res["f32_refined"]
     N = n ** 3
     # algorithmic bytes (S sweeps): FP32 solves (S+1) 2 D N 4, FP64 residuals
-    # S (2 D N 8 + 8 N), FP32 updates S N 4; evaluation nq (24 + 4) + N 4
-    b_build = (sweeps + 1) * 2 * 3 * N * 4 + sweeps * (2 * 3 * N * 8 + 8 * N) + sweeps * N * 4
+    # S (2 D N 8 + 8 N), FP32 updates added on store (S-1) N 4, the gradient
+    # table (c and the last correction in, 16-byte cells out: N (8 + 16))
+    # and the last update (N 12); evaluation nq (24 + 4) + N 4
+    b_build = refined_build_bytes(3, N, sweeps)
     b_eval = nq * (24 + 4) + N * 4
     return {"workload": "3D %d^3 order-5 clamped, %d queries (value); FP32 control points, FP32 results" % (n, nq),
             "sweeps": sweeps, "refined_f32": res["f32_refined"], "f64_form": res["f64"],
@@ -172,6 +174,16 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
 SERVE_RATES = (1e2, 3e2, 1e3, 3e3, 1e4)
 
 
+def refined_build_bytes(dim: int, N: int, sweeps: int) -> int:
+    """Algorithmic HBM bytes of a refined build with S sweeps (as the
+    serving loop counts them): FP32 solves, FP64 residuals, the FP32
+    updates added on store, the gradient table and the last update."""
+    S = sweeps
+    cell = 8 if dim == 1 else 16
+    return ((S + 1) * 2 * dim * N * 4 + S * (2 * dim * N * 8 + 8 * N) + max(S - 1, 0) * N * 4
+            + N * ((8 if S > 0 else 4) + cell) + (12 * N if S > 0 else 0))
+
+
 def measure_refined_headline(amd, torch, steps: int, warmup: int, w, f_d, q_d, out64, f64_ms: float, sweeps: int = 1):
     """The refined-FP32 form on the headline workload (3D 256^3 cubic,
     value + 3 partials at 2^26 points, FP32 results): the same step as the
@@ -181,10 +193,16 @@ def measure_refined_headline(amd, torch, steps: int, warmup: int, w, f_d, q_d, o
     out32 = torch.empty((out64.shape[0], out64.shape[1]), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
-    def step():
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
         s = amd.InterpolationFunction(3, f_d, periodic=w["periodic"], ranges=w["ranges"], precision="f32_refined",
                                       sweeps=sweeps)
-        s.eval(q_d, w["orders"], out=out32.t())
+        if ev is not None:
+            ev[1].record(stream)
+        s.eval(q_d, w["orders"], out=out32.t(), timed=ev is not None)
+        if ev is not None:
+            ev[2].record(stream)
         return s
 
     for _ in range(warmup):
@@ -195,6 +213,18 @@ def measure_refined_headline(amd, torch, steps: int, warmup: int, w, f_d, q_d, o
         step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    # the same step again with events: build, eval call, and the eval's
+    # grouping and kernel (not part of the timed loop above)
+    amd.eval_timings()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for i in range(steps):
+        step(events[i])
+    torch.cuda.synchronize()
+    tim = amd.eval_timings()
+    breakdown = {"build": float(np.mean([e[0].elapsed_time(e[1]) for e in events])),
+                 "eval_call": float(np.mean([e[1].elapsed_time(e[2]) for e in events])),
+                 "eval_grouping": float(np.mean([t[0] for t in tim])),
+                 "eval_kernel": float(np.mean([t[1] for t in tim]))}
     errs = {}
     names = ["value", "d/dx", "d/dy", "d/dz"]
     for o in range(out64.shape[1]):
@@ -205,7 +235,10 @@ def measure_refined_headline(amd, torch, steps: int, warmup: int, w, f_d, q_d, o
     del stream
     return {"workload": "the headline step (3D 256^3 cubic build + 2^26 x value, d/dx, d/dy, d/dz) in the refined-FP32 "
                         "form, %d sweep(s), FP32 results" % sweeps,
-            "ms_per_step": ms, "f64_ms_per_step": f64_ms, "speedup_vs_f64": f64_ms / ms, "error_vs_f64": errs}
+            "ms_per_step": ms, "breakdown_ms": breakdown, "f64_ms_per_step": f64_ms, "speedup_vs_f64": f64_ms / ms,
+            "error_vs_f64": errs,
+            "partials": "value and first partials from the gradient table (FP32 backward differences of the "
+                        "FP64-corrected control points)"}
 
 
 def measure_small(amd, torch, steps: int):

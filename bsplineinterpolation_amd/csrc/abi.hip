@@ -48,6 +48,16 @@ hipError_t launch_eval_single(int dim, int order, bool uniform, bool f32, const 
     return launch_eval(dim, order, false, uniform, f32, p, s);
 }
 
+// value and first partials of a refined build from its gradient table
+hipError_t launch_eval_cells(int dim, int order, bool uniform, const EvalParams& p, hipStream_t s) {
+    switch (dim) {
+        case 1: return eval_cells_d1(order, uniform, p, s);
+        case 2: return eval_cells_d2(order, uniform, p, s);
+        case 3: return eval_cells_d3(order, uniform, p, s);
+        default: return hipErrorNotSupported;
+    }
+}
+
 }  // namespace intp_amd
 
 namespace {
@@ -174,6 +184,9 @@ struct intp_spline_s {
     std::atomic<int> refs{1};
     std::shared_ptr<GridData> grid;
     void* ctrl = nullptr;  // FP64, or FP32 for a refined build
+    // refined builds: the gradient table (Cell1 / Cell3 per control
+    // point), in the allocation of `ctrl` after the FP32 control points
+    void* cells = nullptr;
     int ctrl_bytes = 8;
     int sweeps = -1;       // refinement sweeps of a refined build
     UseTracker uses;       // streams that read `ctrl` (solve and every eval)
@@ -425,12 +438,23 @@ int solve_into(const GridData& g, const double* values, double* ctrl, hipStream_
 
 // Refined-FP32 control points of `values` into `ctrl` (FP32): an FP32
 // solve, then `sweeps` rounds of an FP64 residual r = S f - (M_0 (x) ...) c
-// and an FP32 correction solve added to c.  `scratch` holds N floats and
-// 2 N doubles (N doubles for 1D).
-int refine_into(const GridData& g, const double* values, float* ctrl, int sweeps, unsigned char* scratch,
-                hipStream_t stream) {
+// and an FP32 correction solve added to c; and the gradient table `cells`.
+// The last round's correction d is kept apart: the cells take their
+// differences from c + d in FP64 before c becomes c + d rounded.
+// `scratch` holds N floats and 2 N doubles (N doubles for 1D).
+int refine_into(const GridData& g, const double* values, float* ctrl, void* cells, int sweeps,
+                unsigned char* scratch, hipStream_t stream) {
     int rc = solve_axes(g, true, values, ctrl, nullptr, stream);
     if (rc != INTP_OK) return rc;
+    CellParams cp{};
+    cp.c0 = ctrl;
+    cp.cells = cells;
+    cp.dim = g.dim;
+    cp.total = g.total;
+    for (int d = 0; d < g.dim; ++d) {
+        cp.n[d] = g.setup[d].n;
+        cp.periodic[d] = g.setup[d].periodic ? 1 : 0;
+    }
     float* delta = reinterpret_cast<float*>(scratch);
     double* t[2] = {reinterpret_cast<double*>(scratch + ((g.total * 4 + 255) / 256) * 256), nullptr};
     t[1] = t[0] + ((g.total + 31) / 32) * 32;
@@ -469,8 +493,12 @@ int refine_into(const GridData& g, const double* values, float* ctrl, int sweeps
             in_f32 = false;
         }
         // correction: solve on delta (already shifted), added into ctrl
-        if ((rc = solve_axes(g, true, nullptr, delta, ctrl, stream)) != INTP_OK) return rc;
+        // (the last one by the cells' launch)
+        const bool last_sweep = sweep + 1 == sweeps;
+        if ((rc = solve_axes(g, true, nullptr, delta, last_sweep ? nullptr : ctrl, stream)) != INTP_OK) return rc;
+        if (last_sweep) cp.delta = delta;
     }
+    HIP_TRY(launch_refined_cells(cp, stream), "gradient table launch");
     return INTP_OK;
 }
 
@@ -699,8 +727,11 @@ int interpolate_impl(intp_template t, const double* values, int values_mem, bool
     s->grid = t->grid;
     s->ctrl_bytes = refined ? 4 : 8;
     s->sweeps = refined ? sweeps : -1;
-    HIP_TRY(device_alloc(&s->ctrl, static_cast<size_t>(g.total) * s->ctrl_bytes, stream),
-            "device allocation (control points)");
+    const size_t ctrl_bytes = static_cast<size_t>(g.total) * s->ctrl_bytes;
+    const size_t cells_at = (ctrl_bytes + 255) / 256 * 256;
+    const size_t table_bytes = refined ? cells_at + static_cast<size_t>(g.total) * cell_bytes(g.dim) : ctrl_bytes;
+    HIP_TRY(device_alloc(&s->ctrl, table_bytes, stream), "device allocation (control points)");
+    if (refined) s->cells = static_cast<unsigned char*>(s->ctrl) + cells_at;
     const double* src = values;
     double* staged = nullptr;
     unsigned char* scratch = nullptr;
@@ -719,7 +750,7 @@ int interpolate_impl(intp_template t, const double* values, int values_mem, bool
         if (e != hipSuccess) rc = hip_fail(e, "device allocation (refinement scratch)");
     }
     if (rc == INTP_OK)
-        rc = refined ? refine_into(g, src, static_cast<float*>(s->ctrl), sweeps, scratch, stream)
+        rc = refined ? refine_into(g, src, static_cast<float*>(s->ctrl), s->cells, sweeps, scratch, stream)
                      : solve_into(g, src, static_cast<double*>(s->ctrl), stream);
     // the spline's release waits for the solve; the grid's for its use here
     const hipError_t er = s->uses.record(stream);
@@ -1126,6 +1157,10 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         sc.put(rec_b);
         sc.put(bin_scratch);
     };
+    // refined builds: value and first partials come from the gradient table,
+    // whose elements size the grouping's brick patches
+    const bool cells = any_grad && s->cells != nullptr;
+    const int elem_bytes = cells ? cell_bytes(D) : s->ctrl_bytes;
     const bool group = (flags & INTP_EVAL_GROUPED) ? npts > 0
                        : !(flags & INTP_EVAL_UNGROUPED) && group_pays(D, g.total * s->ctrl_bytes, npts);
     BrickGrid ib{};
@@ -1135,7 +1170,7 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     // request, and for a fixed scratch buffer, where it needs half the
     // memory.
     const bool groupable = fast && D >= 2 && group && npts <= 0xFFFFFFFFll;
-    const bool bricks_ok = groupable && item_bricks(g, s->ctrl_bytes, &ib, &item_patch, &item_bins);
+    const bool bricks_ok = groupable && item_bricks(g, elem_bytes, &ib, &item_patch, &item_bins);
     const bool two_ok = groupable && bin_scan_tiles(npts) <= 4096;
     bool one_pass = bricks_ok && ((flags & INTP_EVAL_GROUP_ONE_PASS) || sc.fixed() || !two_ok);
     bool two_pass = !one_pass && two_ok;
@@ -1204,7 +1239,7 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         p.max_items = gp.max_items;
     } else if (two_pass) {
         int64_t patch_volume = 0;
-        BinParams bp = bin_params(g, s->ctrl_bytes, &patch_volume);
+        BinParams bp = bin_params(g, elem_bytes, &patch_volume);
         bp.coords = p.coords;
         bp.cs_dim = p.cs_dim;
         bp.cs_pt = p.cs_pt;
@@ -1235,7 +1270,15 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     }
     if (timing && e == hipSuccess) e = hipEventRecord(timed.ev[1], stream);
     const bool f32 = s->ctrl_bytes == 4;
-    if (any_grad && e == hipSuccess) e = launch_eval_grad(D, g.order, fast, f32, p, stream);
+    if (any_grad && e == hipSuccess) {
+        if (cells) {
+            EvalParams pc = p;
+            pc.ctrl = s->cells;
+            e = launch_eval_cells(D, g.order, fast, pc, stream);
+        } else {
+            e = launch_eval_grad(D, g.order, fast, f32, p, stream);
+        }
+    }
     for (size_t i = 0; e == hipSuccess && i < single.size(); ++i) {
         for (int d = 0; d < D; ++d) p.order[d] = orders[single[i] * D + d];
         p.out_index = single[i];

@@ -1,0 +1,11 @@
+// eval_cells_d2.hip -- gradient kernels for 2D refined builds on their
+// gradient table (value and first partials from the FP32 cells).
+#include "eval_impl.hpp"
+
+namespace intp_amd {
+
+hipError_t eval_cells_d2(int order, bool uniform, const EvalParams& p, hipStream_t s) {
+    return eval_dispatch<2, Cell3>(order, true, uniform, p, s);
+}
+
+}  // namespace intp_amd

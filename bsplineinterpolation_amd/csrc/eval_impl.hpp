@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.hpp"
 #include "span.hpp"
@@ -41,9 +42,46 @@ constexpr int kEvalBlock = 256;
 template <int K, typename Off, typename W>
 struct DimW {
     W w[K + 1];      // value weights (or the single output's weights)
-    W dw[K + 1];     // first-derivative weights (gradient kernels)
+    W dw[K + 1];     // first-derivative weights (gradient kernels); on a
+                     // gradient table, the weights of the differences
     Off off[K + 1];  // element offsets of the K+1 stencil planes
 };
+
+// Weight and result type of a table element type: the element type
+// itself, or float for the cells of a refined build's gradient table.
+template <typename T>
+struct TableWeight {
+    using type = T;
+};
+template <>
+struct TableWeight<Cell1> {
+    using type = float;
+};
+template <>
+struct TableWeight<Cell3> {
+    using type = float;
+};
+template <typename T>
+using weight_t = typename TableWeight<T>::type;
+template <typename T>
+constexpr bool is_cells() {
+    return !std::is_same<T, weight_t<T>>::value;
+}
+
+// Weights of the backward differences from the first-derivative weights
+// dw of one dimension: sum_m dw[m] c[m] = sum_m g[m] (c[m] - c[m-1]) with
+// g[m] = -(dw[0] + ... + dw[m-1]) (summation by parts; the dw sum to 0),
+// so g[0] = 0 and g[m] = k B_{k-1}[m] / (t[mu+m] - t[mu-k+m]) >= 0, the
+// scaled order-(k-1) basis of the derivative identity.  Summed in FP64.
+template <int K, typename W>
+__device__ __forceinline__ void difference_weights(const double (&dw)[K + 1], W (&g)[K + 1]) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m <= K; ++m) {
+        g[m] = static_cast<W>(-s);
+        s += dw[m];
+    }
+}
 
 // ---------------------------------------------------------------------------
 // span search
@@ -155,7 +193,7 @@ __device__ __forceinline__ void stencil_offsets(const DimParams& dp, int mu, Off
 
 // General grids: weights of the value (w) and, if `grad`, of d/dx (dw);
 // otherwise w holds the r-th derivative.
-template <int K, bool GRAD, typename W>
+template <int K, bool GRAD, typename W, bool DIFF = false>
 __device__ __forceinline__ void general_weights(const DimParams& dp, double x, int r, int* bad,
                                                 DimW<K, int64_t, W>& dd) {
     wrap_or_check(dp, x, bad);
@@ -172,7 +210,13 @@ __device__ __forceinline__ void general_weights(const DimParams& dp, double x, i
         for (int m = 0; m <= K; ++m) dd.w[m] = static_cast<W>(b[m]);
         if (K >= 1) differentiate<K>(keep, 1, R);
 #pragma unroll
-        for (int m = 0; m <= K; ++m) dd.dw[m] = static_cast<W>(K >= 1 ? keep[m] : 0.0);
+        for (int m = 0; m <= K; ++m) keep[m] = K >= 1 ? keep[m] : 0.0;
+        if constexpr (DIFF) {
+            difference_weights<K, W>(keep, dd.dw);
+        } else {
+#pragma unroll
+            for (int m = 0; m <= K; ++m) dd.dw[m] = static_cast<W>(keep[m]);
+        }
     } else {
         if (r == 0) {
 #pragma unroll
@@ -203,10 +247,11 @@ __device__ __forceinline__ int uniform_span(const DimParams& dp, double x) {
 // Weights from the span's basis polynomials P (row j: coefficients of
 // basis j in u = (x - t_mu) / dx, ascending powers); PP is a pointer to
 // the coefficients in a given address space.
-template <int K, bool GRAD, typename W, typename PP>
+template <int K, bool GRAD, typename W, bool DIFF, typename PP>
 __device__ __forceinline__ void poly_weights(PP P, double u, double inv_dx, int r, DimW<K, uint32_t, W>& dd) {
     constexpr int NW = K + 1;
     if constexpr (GRAD) {
+        double sum = 0.0;  // DIFF: running sum of the derivative weights (difference_weights)
 #pragma unroll
         for (int j = 0; j <= K; ++j) {
             double v = P[j * NW + K];
@@ -217,7 +262,13 @@ __device__ __forceinline__ void poly_weights(PP P, double u, double inv_dx, int 
                 if (q >= 1) g = g * u + static_cast<double>(q) * P[j * NW + q];
             }
             dd.w[j] = static_cast<W>(v);
-            dd.dw[j] = static_cast<W>(K >= 1 ? g * inv_dx : 0.0);
+            const double dw = K >= 1 ? g * inv_dx : 0.0;
+            if constexpr (DIFF) {
+                dd.dw[j] = static_cast<W>(-sum);
+                sum += dw;
+            } else {
+                dd.dw[j] = static_cast<W>(dw);
+            }
         }
     } else if (r == 0) {
         // the value: Horner
@@ -256,7 +307,7 @@ __device__ __forceinline__ void poly_weights(PP P, double u, double inv_dx, int 
 // common case), otherwise with vector loads of each lane's class (a few KB,
 // cached).  No LDS copy: staging the classes in every workgroup cost more
 // than it saved (up to 9.5 KB per workgroup at order 5).
-template <int K, bool GRAD, typename W>
+template <int K, bool GRAD, typename W, bool DIFF = false>
 __device__ __forceinline__ int uniform_weights(const DimParams& dp, double x, int r, int* bad,
                                                DimW<K, uint32_t, W>& dd) {
     wrap_or_check(dp, x, bad);
@@ -267,10 +318,10 @@ __device__ __forceinline__ int uniform_weights(const DimParams& dp, double x, in
     // explicit address spaces: as one generic pointer both branches would
     // read through flat loads (and the uniform case without scalar loads)
     if (__all(cls == 0))
-        poly_weights<K, GRAD, W>((const __attribute__((address_space(4))) double*)dp.poly, u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W, DIFF>((const __attribute__((address_space(4))) double*)dp.poly, u, dp.inv_dx, r, dd);
     else
-        poly_weights<K, GRAD, W>((const __attribute__((address_space(1))) double*)(dp.poly + cls * (K + 1) * (K + 1)),
-                                 u, dp.inv_dx, r, dd);
+        poly_weights<K, GRAD, W, DIFF>(
+            (const __attribute__((address_space(1))) double*)(dp.poly + cls * (K + 1) * (K + 1)), u, dp.inv_dx, r, dd);
     stencil_offsets<K>(dp, mu, dd.off);
     return mu;
 }
@@ -282,8 +333,8 @@ __device__ __forceinline__ int uniform_weights(const DimParams& dp, double x, in
 // One stencil row along the last dimension.  LASTC: the last dimension is
 // not periodic, so the row is K+1 consecutive elements from one address
 // (one address computation, immediate offsets).
-template <int K, bool LASTC, typename T, typename Off>
-__device__ __forceinline__ void load_row(const T* __restrict__ c, Off rowoff, const DimW<K, Off, T>& dl,
+template <int K, bool LASTC, typename T, typename Off, typename W>
+__device__ __forceinline__ void load_row(const T* __restrict__ c, Off rowoff, const DimW<K, Off, W>& dl,
                                          T (&cv)[K + 1]) {
     if constexpr (LASTC) {
         const T* __restrict__ r = c + (rowoff + dl.off[0]);
@@ -413,6 +464,90 @@ __device__ __forceinline__ T contract_single(const T* __restrict__ c, DimW<K, Of
     return v;
 }
 
+// Value and first partials on a gradient table (cells {c, e_0, .., e_{D-1}}):
+// the value contracts c with the value weights w, partial d contracts e_d
+// with the difference weights (dw) along d and w along the others.  Same
+// axis order as contract_grad.
+template <int D, int K, bool LASTC, typename T, typename Off>
+__device__ __forceinline__ void contract_cells(const T* __restrict__ c, DimW<K, Off, float> (&dd)[D],
+                                               float (&res)[D + 1]) {
+    if constexpr (D == 1) {
+        T cv[K + 1];
+        load_row<K, LASTC, T>(c, Off(0), dd[0], cv);
+        float v = 0.f, g = 0.f;
+#pragma unroll
+        for (int a = 0; a <= K; ++a) {
+            v += dd[0].w[a] * cv[a].c;
+            g += dd[0].dw[a] * cv[a].e0;
+        }
+        res[0] = v;
+        res[1] = g;
+    } else if constexpr (D == 2) {
+        float v = 0.f, g0 = 0.f, g1 = 0.f;
+#pragma unroll
+        for (int a = 0; a <= K; ++a) {
+            T cv[K + 1];
+            load_row<K, LASTC, T>(c, dd[0].off[a], dd[1], cv);
+            float sC = 0.f, s0 = 0.f, s1 = 0.f;
+#pragma unroll
+            for (int b = 0; b <= K; ++b) {
+                sC += dd[1].w[b] * cv[b].c;
+                s0 += dd[1].w[b] * cv[b].e0;
+                s1 += dd[1].dw[b] * cv[b].e1;
+            }
+            v += dd[0].w[a] * sC;
+            g0 += dd[0].dw[a] * s0;
+            g1 += dd[0].w[a] * s1;
+        }
+        res[0] = v;
+        res[1] = g0;
+        res[2] = g1;
+    } else {
+        float v = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+        constexpr int kOuterUnroll = K <= INTP_OUTER_UNROLL_MAX_K ? K + 1 : 1;
+#pragma unroll kOuterUnroll
+        for (int a = 0; a <= K; ++a) {
+            float tC = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
+#pragma unroll
+            for (int b = 0; b <= K; ++b) {
+                T cv[K + 1];
+                load_row<K, LASTC, T>(c, dd[0].off[0] + dd[1].off[b], dd[2], cv);
+                float sC = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+                for (int e = 0; e <= K; ++e) {
+                    sC += dd[2].w[e] * cv[e].c;
+                    s0 += dd[2].w[e] * cv[e].e0;
+                    s1 += dd[2].w[e] * cv[e].e1;
+                    s2 += dd[2].dw[e] * cv[e].e2;
+                }
+                tC += dd[1].w[b] * sC;
+                t0 += dd[1].w[b] * s0;
+                t1 += dd[1].dw[b] * s1;
+                t2 += dd[1].w[b] * s2;
+            }
+            v += dd[0].w[0] * tC;
+            g0 += dd[0].dw[0] * t0;
+            g1 += dd[0].w[0] * t1;
+            g2 += dd[0].w[0] * t2;
+            shift_down<K, Off, float>(dd[0]);
+        }
+        res[0] = v;
+        res[1] = g0;
+        res[2] = g1;
+        res[3] = g2;
+    }
+}
+
+// The gradient kernels' contraction for a table of element type T.
+template <int D, int K, bool LASTC, typename T, typename Off>
+__device__ __forceinline__ void contract_outputs(const T* __restrict__ c, DimW<K, Off, weight_t<T>> (&dd)[D],
+                                                 weight_t<T> (&res)[D + 1]) {
+    if constexpr (is_cells<T>())
+        contract_cells<D, K, LASTC, T, Off>(c, dd, res);
+    else
+        contract_grad<D, K, LASTC, T, Off>(c, dd, res);
+}
+
 // Stores the value and first partials of one point.
 // One output value (FP64, or FP32 into out32).
 __device__ __forceinline__ void store_out(const EvalParams& p, int64_t at, double v) {
@@ -447,19 +582,21 @@ __device__ __forceinline__ int64_t block_index() { return xcd_chunk_index(); }
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void general_point(const EvalParams& p, int64_t pt) {
+    using W = weight_t<T>;
     const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
-    DimW<K, int64_t, T> dd[D];
+    DimW<K, int64_t, W> dd[D];
     bool zero = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        general_weights<K, GRAD, T>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+        general_weights<K, GRAD, W, is_cells<T>()>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad,
+                                                   dd[d]);
     }
     if constexpr (GRAD) {
-        T res[D + 1];
-        contract_grad<D, K, LASTC>(ctrl, dd, res);
+        W res[D + 1];
+        contract_outputs<D, K, LASTC>(ctrl, dd, res);
 #pragma unroll
         for (int s = 0; s <= D; ++s)
             if (p.slot[s] >= 0) store_out(p, p.slot[s] * p.os_out + opt * p.os_pt, res[s]);
@@ -477,19 +614,21 @@ __global__ __launch_bounds__(kEvalBlock) void eval_general_kernel(EvalParams p) 
 
 template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void uniform_point(const EvalParams& p, int64_t pt) {
+    using W = weight_t<T>;
     const int64_t opt = p.perm ? static_cast<int64_t>(p.perm[pt * p.perm_stride]) : pt;
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
-    DimW<K, uint32_t, T> dd[D];
+    DimW<K, uint32_t, W> dd[D];
     bool zero = false;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        uniform_weights<K, GRAD, T>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad, dd[d]);
+        uniform_weights<K, GRAD, W, is_cells<T>()>(p.dim[d], p.coords[d * p.cs_dim + pt * p.cs_pt], r, p.bad,
+                                                   dd[d]);
     }
     if constexpr (GRAD) {
-        T res[D + 1];
-        contract_grad<D, K, LASTC>(ctrl, dd, res);
+        W res[D + 1];
+        contract_outputs<D, K, LASTC>(ctrl, dd, res);
         store_grad<D>(p, opt, res);
     } else {
         const T v = contract_single<D, K, LASTC>(ctrl, dd);
@@ -530,17 +669,17 @@ constexpr int patch_max() {
 template <int D, int K, bool GRAD, bool LASTC, typename T>
 __device__ __forceinline__ void patch_point(const EvalParams& p, const T* s_patch, const int (&bstart)[D], const int (&bsize)[D],
                                             const uint32_t (&lstride)[D], const BinRecord& rec) {
+    using W = weight_t<T>;
     const int64_t opt = static_cast<int64_t>(rec.idx);
     const T* __restrict__ ctrl = static_cast<const T*>(p.ctrl);
-    DimW<K, uint32_t, T> dd[D];
+    DimW<K, uint32_t, W> dd[D];
     bool zero = false, inside = true;
     int local[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const int r = GRAD ? 1 : p.order[d];
         zero = zero || r > K;
-        const int mu = uniform_weights<K, GRAD, T>(p.dim[d], rec.x[d], r,
-                                                   p.bad, dd[d]);
+        const int mu = uniform_weights<K, GRAD, W, is_cells<T>()>(p.dim[d], rec.x[d], r, p.bad, dd[d]);
         const int n = static_cast<int>(p.dim[d].n);
         int st = mu - K;  // first stencil point; periodic even orders: may be n (wraps to 0)
         if (st >= n) st -= n;
@@ -549,19 +688,19 @@ __device__ __forceinline__ void patch_point(const EvalParams& p, const T* s_patc
         local[d] = l;
         inside = inside && l >= 0 && l < bsize[d];
     }
-    T res[D + 1];
+    W res[D + 1];
     if (inside) {
 #pragma unroll
         for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int m = 0; m <= K; ++m) dd[d].off[m] = static_cast<uint32_t>(local[d] + m) * lstride[d];
         if constexpr (GRAD)
-            contract_grad<D, K, true>(s_patch, dd, res);
+            contract_outputs<D, K, true>(s_patch, dd, res);
         else
             res[0] = contract_single<D, K, true>(s_patch, dd);
     } else {
         if constexpr (GRAD)
-            contract_grad<D, K, LASTC>(ctrl, dd, res);
+            contract_outputs<D, K, LASTC>(ctrl, dd, res);
         else
             res[0] = contract_single<D, K, LASTC>(ctrl, dd);
     }
@@ -652,7 +791,7 @@ __global__ __launch_bounds__(kEvalBlock) void eval_patch_kernel(EvalParams p) {
 #pragma unroll
             for (int r = 0; r < kRows; ++r) {
                 const int i0 = group + r * kGroups;
-                v[r] = T(0);
+                v[r] = T{};
                 if (i0 < psize[0]) {
                     int g0 = bstart[0] + i0;
                     if (g0 >= n0) g0 = per0 ? g0 % n0 : -1;
@@ -793,8 +932,11 @@ hipError_t launch_items(const EvalParams& p, size_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Gradient tables (cells) have gradient kernels only.
 template <int D, int K, bool LASTC, typename T>
 hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    constexpr bool kSingle = !is_cells<T>();
+    if (!grad && !kSingle) return hipErrorInvalidValue;
     if (uniform && p.items) {
         // brick items of the one-pass grouping (2D and 3D grids)
         if constexpr (D >= 2) {
@@ -802,7 +944,7 @@ hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStrea
                 return hipErrorInvalidValue;
             const size_t lds = static_cast<size_t>(p.patch_volume) * sizeof(T);
             if (grad) return launch_items<eval_brick_kernel<D, K, true, LASTC, T>>(p, lds, s);
-            return launch_items<eval_brick_kernel<D, K, false, LASTC, T>>(p, lds, s);
+            if constexpr (kSingle) return launch_items<eval_brick_kernel<D, K, false, LASTC, T>>(p, lds, s);
         }
         return hipErrorInvalidValue;
     }
@@ -812,14 +954,17 @@ hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStrea
             p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
             return hipErrorInvalidValue;
         if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC, T>>(p, s);
-        return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
+        if constexpr (kSingle) return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
+        return hipErrorInvalidValue;
     }
     if (uniform) {
         if (grad) return launch_chunks<eval_uniform_kernel<D, K, true, LASTC, T>>(p, s);
-        return launch_chunks<eval_uniform_kernel<D, K, false, LASTC, T>>(p, s);
+        if constexpr (kSingle) return launch_chunks<eval_uniform_kernel<D, K, false, LASTC, T>>(p, s);
+        return hipErrorInvalidValue;
     }
     if (grad) return launch_chunks<eval_general_kernel<D, K, true, LASTC, T>>(p, s);
-    return launch_chunks<eval_general_kernel<D, K, false, LASTC, T>>(p, s);
+    if constexpr (kSingle) return launch_chunks<eval_general_kernel<D, K, false, LASTC, T>>(p, s);
+    return hipErrorInvalidValue;
 }
 
 template <int D, int K, typename T>

@@ -52,7 +52,7 @@ struct BrickItem {
 
 struct EvalParams {
     DimParams dim[kMaxDim];
-    const void* ctrl;  // control points: double, or float for a refined build
+    const void* ctrl;  // control points: double, float for a refined build, or its gradient table
     const double* coords;
     int64_t cs_dim, cs_pt;
     double* out;
@@ -315,5 +315,39 @@ struct MatvecParams {
 };
 hipError_t launch_band_matvec(const MatvecParams& p, bool in_f32, bool out_f32, hipStream_t s);
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s);
+
+// Gradient table of a refined build: per control point, its FP32 value c
+// and its backward differences e_d = c - c[i_d - 1] along each dimension
+// (wrapped on periodic dimensions, 0 at i_d = 0 of a clamped one), formed
+// in FP64 from the FP64-corrected control points and then rounded.  A
+// first partial is the contraction of the differences with the
+// nonnegative order-(k-1) weights (de Boor's derivative identity,
+// BSpline.hpp:489-520 differentiates the same way), so FP32 rounding of
+// the table costs the partials no more than it costs the value.
+struct alignas(8) Cell1 {
+    float c, e0;
+};
+struct alignas(16) Cell3 {
+    float c, e0, e1, e2;  // 2D tables leave e2 at 0
+};
+inline int cell_bytes(int dim) { return dim == 1 ? static_cast<int>(sizeof(Cell1)) : static_cast<int>(sizeof(Cell3)); }
+
+struct CellParams {
+    const float* c0;     // FP32 control points
+    const float* delta;  // FP32 correction (c = c0 + delta in FP64), or null
+    void* cells;         // Cell1 (1D) or Cell3 (2D, 3D)
+    int32_t dim;
+    int64_t n[kMaxDim];
+    int32_t periodic[kMaxDim];
+    int64_t total;
+    FastDiv div_n[kMaxDim];  // set by launch_refined_cells
+};
+// cells from (c0, delta); then c0 += delta rounded once from FP64.
+// Tables below 2^32 elements.
+hipError_t launch_refined_cells(const CellParams& p, hipStream_t s);
+// gradient kernels on a gradient table (eval_cells_d*.hip)
+hipError_t eval_cells_d1(int order, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_cells_d2(int order, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_cells_d3(int order, bool uniform, const EvalParams& p, hipStream_t s);
 
 }  // namespace intp_amd

@@ -150,6 +150,53 @@ __global__ __launch_bounds__(kBlock) void add_f32_kernel(float* __restrict__ c, 
     if (e < total) c[e] = c[e] + d[e];
 }
 
+// Gradient-table cells of element e: c = c0 + delta in FP64 (exact: the
+// correction is far below c0's last bit), the backward differences of
+// those sums along each dimension, each rounded once to FP32.
+template <int D, typename C>
+__global__ __launch_bounds__(kBlock) void refined_cells_kernel(CellParams p) {
+    const uint64_t e64 = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (e64 >= static_cast<uint64_t>(p.total)) return;
+    const uint32_t e = static_cast<uint32_t>(e64);
+    const float* __restrict__ c0 = p.c0;
+    const float* __restrict__ delta = p.delta;
+    auto full = [&](uint32_t i) {
+        double v = static_cast<double>(c0[i]);
+        if (delta) v += static_cast<double>(delta[i]);
+        return v;
+    };
+    const double c = full(e);
+    float diff[3] = {0.f, 0.f, 0.f};
+    uint32_t rem = e, stride = 1;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+        const uint32_t nd = static_cast<uint32_t>(p.n[d]);
+        const uint32_t q = fast_div(rem, p.div_n[d]);
+        const uint32_t id = rem - q * nd;
+        rem = q;
+        if (id > 0)
+            diff[d] = static_cast<float>(c - full(e - stride));
+        else if (p.periodic[d])
+            diff[d] = static_cast<float>(c - full(e + (nd - 1) * stride));
+        stride *= nd;
+    }
+    C out;
+    out.c = static_cast<float>(c);
+    out.e0 = diff[0];
+    if constexpr (D >= 2) {
+        out.e1 = diff[1];
+        out.e2 = diff[2];
+    }
+    static_cast<C*>(p.cells)[e] = out;
+}
+
+// c0 = c0 + delta, summed in FP64 and rounded once (the cells' c).
+__global__ __launch_bounds__(kBlock) void add_rounded_kernel(float* __restrict__ c, const float* __restrict__ d,
+                                                             int64_t total) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (e < total) c[e] = static_cast<float>(static_cast<double>(c[e]) + static_cast<double>(d[e]));
+}
+
 constexpr int kRunRows = 8;
 
 template <int W>
@@ -211,6 +258,30 @@ hipError_t launch_band_matvec(const MatvecParams& p_in, bool in_f32, bool out_f3
         case 6: return launch_matvec_width<6>(p, in_f32, out_f32, blocks, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_refined_cells(const CellParams& p_in, hipStream_t s) {
+    if (p_in.total >= (int64_t{1} << 32) || p_in.dim < 1 || p_in.dim > kMaxDim) return hipErrorInvalidValue;
+    CellParams p = p_in;
+    int64_t total = 1;
+    for (int d = 0; d < p.dim; ++d) {
+        p.div_n[d] = make_fast_div(static_cast<uint32_t>(p.n[d]));
+        total *= p.n[d];
+    }
+    if (total != p.total) return hipErrorInvalidValue;
+    const int64_t blocks = (p.total + kBlock - 1) / kBlock;
+    if (blocks == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(blocks)), block(kBlock);
+    switch (p.dim) {
+        case 1: hipLaunchKernelGGL((refined_cells_kernel<1, Cell1>), grid, block, 0, s, p); break;
+        case 2: hipLaunchKernelGGL((refined_cells_kernel<2, Cell3>), grid, block, 0, s, p); break;
+        default: hipLaunchKernelGGL((refined_cells_kernel<3, Cell3>), grid, block, 0, s, p); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !p.delta) return e;
+    // the cells read c0 around each element: update it only afterwards
+    hipLaunchKernelGGL(add_rounded_kernel, grid, block, 0, s, const_cast<float*>(p.c0), p.delta, p.total);
+    return hipGetLastError();
 }
 
 hipError_t launch_add_f32(float* c, const float* d, int64_t total, hipStream_t s) {

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "../../include/intp_amd.h"
+#include "kernels.hpp"
 #include "runtime.hpp"
 
 using namespace intp_amd;
@@ -82,29 +83,33 @@ struct Resident {
     std::list<int64_t>::iterator lru;
 };
 
-// Control-point bytes a build keeps resident, plus what a refined build
-// needs while it runs (FP32 correction and two FP64 residual buffers).
-int64_t build_footprint(const intp_grid_desc& d, int precision) {
-    int64_t n = 1;
-    for (int i = 0; i < d.dim; ++i) n *= d.n[i];
-    return precision == INTP_PREC_F32_REFINED ? n * 4 + n * (4 + 16) : n * 8;
-}
-
+// Table bytes a build keeps resident (refined: FP32 control points and
+// the gradient table), plus what a refined build needs while it runs
+// (FP32 correction and two FP64 residual buffers).
 int64_t resident_bytes_of(const intp_grid_desc& d, int precision) {
     int64_t n = 1;
     for (int i = 0; i < d.dim; ++i) n *= d.n[i];
-    return n * (precision == INTP_PREC_F32_REFINED ? 4 : 8);
+    return n * (precision == INTP_PREC_F32_REFINED ? 4 + cell_bytes(d.dim) : 8);
+}
+
+int64_t build_footprint(const intp_grid_desc& d, int precision) {
+    int64_t n = 1;
+    for (int i = 0; i < d.dim; ++i) n *= d.n[i];
+    return resident_bytes_of(d, precision) + (precision == INTP_PREC_F32_REFINED ? n * (4 + 16) : 0);
 }
 
 // Algorithmic HBM bytes of a build (2 D N s per solve; refined: S+1 FP32
-// solves, S FP64 residuals with the subtraction of f, S FP32 updates).
+// solves, S FP64 residuals with the subtraction of f, S-1 FP32 updates
+// added on store, the gradient table from c (and the last correction),
+// and the last update).
 double build_bytes(const intp_grid_desc& d, int precision, int sweeps) {
     double n = 1;
     for (int i = 0; i < d.dim; ++i) n *= static_cast<double>(d.n[i]);
     const double D = d.dim;
     if (precision != INTP_PREC_F32_REFINED) return 2 * D * n * 8;
     const double S = sweeps;
-    return (S + 1) * 2 * D * n * 4 + S * (2 * D * n * 8 + 8 * n) + S * n * 4;
+    const double cells = n * ((S > 0 ? 8 : 4) + cell_bytes(d.dim)) + (S > 0 ? 12 * n : 0);
+    return (S + 1) * 2 * D * n * 4 + S * (2 * D * n * 8 + 8 * n) + (S > 0 ? S - 1 : 0) * n * 4 + cells;
 }
 
 }  // namespace

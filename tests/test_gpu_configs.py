@@ -95,6 +95,39 @@ def test_gpu_config2_3d256_cubic_full_size(amd, orc, datagen):
     _check(got, o.eval(qs, w["orders"]), TOL64, "3D 256^3 cubic, 2^26 queries")
 
 
+def test_gpu_config2_refined_full_size(amd, orc, datagen):
+    """The headline workload in the refined-FP32 form (one sweep, FP32
+    results): value and the three partials within 1e-6 of the FP64 results
+    at all 2^26 points (max and norm-wise), and a subsample within 1e-6 of
+    the FP64 oracle."""
+    import torch
+    w = datagen.workload_3d_256_cubic()
+    nq = w["queries"].shape[1]
+    f_d = torch.from_numpy(w["f"]).cuda()
+    q_d = torch.from_numpy(np.ascontiguousarray(w["queries"].T)).cuda()
+    del w["queries"]
+    print("config 2 refined: FP64 and refined builds, 2^26 points (value + 3 partials)", flush=True)
+    g64 = amd.InterpolationFunction(3, f_d, periodic=w["periodic"], ranges=w["ranges"])
+    out64 = torch.empty((nq, 4), dtype=torch.float64, device="cuda")
+    g64.eval(q_d, w["orders"], out=out64.t())
+    g = amd.InterpolationFunction(3, f_d, periodic=w["periodic"], ranges=w["ranges"], precision="f32_refined",
+                                  sweeps=1)
+    out32 = torch.empty((nq, 4), dtype=torch.float32, device="cuda")
+    g.eval(q_d, w["orders"], out=out32.t())
+    torch.cuda.synchronize()
+    for i in range(4):
+        ref = out64[:, i]
+        d = out32[:, i].double() - ref
+        e_max = float(d.abs().max() / ref.abs().max())
+        e_norm = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+        print("refined vs FP64, output %d: norm %.3g max %.3g" % (i, e_norm, e_max))
+        assert e_max < TOL_REFINED and e_norm < TOL_REFINED, (i, e_norm, e_max)
+    o = orc.OracleSpline(3, w["f"], periodic=w["periodic"], ranges=w["ranges"])
+    idx = torch.from_numpy(_subsample(nq, 1 << 14, 0xF32)).cuda()
+    got = out32[idx].double().cpu().numpy().T
+    _check(got, o.eval(q_d[idx].cpu().numpy(), w["orders"]), TOL_REFINED, "3D 256^3 cubic refined")
+
+
 def test_gpu_config3_batch_4096_mixed(amd, orc, datagen):
     import torch
     w = datagen.workload_batch_mixed()
