@@ -11,7 +11,7 @@ import os
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libintp_amd.so")
 
-MAX_DIM = 3
+MAX_DIM = 6
 MAX_ORDER = 7
 
 INTP_OK = 0

@@ -24,7 +24,8 @@ ARCH = os.environ.get("INTP_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["abi.hip", "batch.hip", "bin.hip", "group.hip", "proxy.hip", "refine.hip", "serve.hip", "solve.hip"] + [
     "batch_eval_d%d.hip" % d for d in (1, 2, 3)] + [
-    "eval_%s_d%d.hip" % (t, d) for t in ("f64", "f32", "cells") for d in (1, 2, 3)]
+    "eval_%s_d%d.hip" % (t, d) for t in ("f64", "f32") for d in (1, 2, 3, 4, 5, 6)] + [
+    "eval_cells_d%d.hip" % d for d in (1, 2, 3)]
 CXX_SOURCES = ["setup.cpp", "host_eval.cpp"]
 
 HIPCC_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall",

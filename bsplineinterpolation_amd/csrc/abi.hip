@@ -36,6 +36,12 @@ hipError_t launch_eval(int dim, int order, bool grad, bool uniform, bool f32, co
         case 5: return eval_dispatch_f32_d2(order, grad, uniform, p, s);
         case 6: return eval_dispatch_f64_d3(order, grad, uniform, p, s);
         case 7: return eval_dispatch_f32_d3(order, grad, uniform, p, s);
+        case 8: return eval_dispatch_f64_d4(order, grad, uniform, p, s);
+        case 9: return eval_dispatch_f32_d4(order, grad, uniform, p, s);
+        case 10: return eval_dispatch_f64_d5(order, grad, uniform, p, s);
+        case 11: return eval_dispatch_f32_d5(order, grad, uniform, p, s);
+        case 12: return eval_dispatch_f64_d6(order, grad, uniform, p, s);
+        case 13: return eval_dispatch_f32_d6(order, grad, uniform, p, s);
         default: return hipErrorNotSupported;
     }
 }
@@ -494,11 +500,11 @@ int refine_into(const GridData& g, const double* values, float* ctrl, void* cell
         }
         // correction: solve on delta (already shifted), added into ctrl
         // (the last one by the cells' launch)
-        const bool last_sweep = sweep + 1 == sweeps;
-        if ((rc = solve_axes(g, true, nullptr, delta, last_sweep ? nullptr : ctrl, stream)) != INTP_OK) return rc;
-        if (last_sweep) cp.delta = delta;
+        const bool keep = sweep + 1 == sweeps && cells != nullptr;
+        if ((rc = solve_axes(g, true, nullptr, delta, keep ? nullptr : ctrl, stream)) != INTP_OK) return rc;
+        if (keep) cp.delta = delta;
     }
-    HIP_TRY(launch_refined_cells(cp, stream), "gradient table launch");
+    if (cells) HIP_TRY(launch_refined_cells(cp, stream), "gradient table launch");
     return INTP_OK;
 }
 
@@ -563,7 +569,7 @@ BinParams bin_params(const GridData& g, int elem_bytes, int64_t* patch_volume) {
 // dimension, and a last-dimension patch row no longer than the kernel's
 // workgroup).  False when that leaves more than kGroupMaxBins bricks.
 bool item_bricks(const GridData& g, int elem_bytes, BrickGrid* out, int64_t* patch_volume, int32_t* nbins) {
-    int shift[kMaxDim] = {0, 0, 0};
+    int shift[kGroupDim] = {0, 0, 0};
     auto volume = [&]() {
         int64_t v = 1;
         for (int d = 0; d < g.dim; ++d) v *= (int64_t{1} << shift[d]) + g.order;
@@ -729,9 +735,11 @@ int interpolate_impl(intp_template t, const double* values, int values_mem, bool
     s->sweeps = refined ? sweeps : -1;
     const size_t ctrl_bytes = static_cast<size_t>(g.total) * s->ctrl_bytes;
     const size_t cells_at = (ctrl_bytes + 255) / 256 * 256;
-    const size_t table_bytes = refined ? cells_at + static_cast<size_t>(g.total) * cell_bytes(g.dim) : ctrl_bytes;
+    // gradient tables for 1 to 3 dimensions (cells of up to 4 floats)
+    const bool with_cells = refined && g.dim <= kCellMaxDim;
+    const size_t table_bytes = with_cells ? cells_at + static_cast<size_t>(g.total) * cell_bytes(g.dim) : ctrl_bytes;
     HIP_TRY(device_alloc(&s->ctrl, table_bytes, stream), "device allocation (control points)");
-    if (refined) s->cells = static_cast<unsigned char*>(s->ctrl) + cells_at;
+    if (with_cells) s->cells = static_cast<unsigned char*>(s->ctrl) + cells_at;
     const double* src = values;
     double* staged = nullptr;
     unsigned char* scratch = nullptr;
@@ -1169,7 +1177,7 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
     // Two radix passes by default (measured faster at 256^3); one pass on
     // request, and for a fixed scratch buffer, where it needs half the
     // memory.
-    const bool groupable = fast && D >= 2 && group && npts <= 0xFFFFFFFFll;
+    const bool groupable = fast && D >= 2 && D <= kGroupDim && group && npts <= 0xFFFFFFFFll;
     const bool bricks_ok = groupable && item_bricks(g, elem_bytes, &ib, &item_patch, &item_bins);
     const bool two_ok = groupable && bin_scan_tiles(npts) <= 4096;
     bool one_pass = bricks_ok && ((flags & INTP_EVAL_GROUP_ONE_PASS) || sc.fixed() || !two_ok);

@@ -87,7 +87,7 @@ __device__ __forceinline__ void batch_solve_item(const BatchSpline* specs, const
             sp.src = values + b.offset;
             sp.outer_dims = a;
 #pragma unroll
-            for (int d = 0; d < kMaxDim; ++d) {
+            for (int d = 0; d < kBatchDim; ++d) {
                 if (d >= a) break;
                 sp.outer_n[d] = b.ep.dim[d].n;
                 sp.outer_shift[d] = b.shift[d];
@@ -465,7 +465,7 @@ struct intp_batch_s {
     int device = 0;
     int count = 0;
     int num_cus = 256;
-    std::vector<DimSetup> setups;  // count * kMaxDim
+    std::vector<DimSetup> setups;  // count * kBatchDim
     std::vector<int> dims, orders;
     std::vector<int64_t> offsets;  // count + 1
     std::vector<unsigned char> image;
@@ -568,9 +568,9 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
         BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
         const int D = b->dims[static_cast<size_t>(s)];
         for (int d = 0; d < D; ++d) {
-            const int64_t lpb = tile_fit(b->setups[static_cast<size_t>(s) * kMaxDim + d].n, b->tune.tile_cap);
+            const int64_t lpb = tile_fit(b->setups[static_cast<size_t>(s) * kBatchDim + d].n, b->tune.tile_cap);
             int64_t inner = 1;
-            for (int e = d + 1; e < D; ++e) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + e].n;
+            for (int e = d + 1; e < D; ++e) inner *= b->setups[static_cast<size_t>(s) * kBatchDim + e].n;
             if (lpb == 0 && inner < 64 && D > 1)
                 return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": line too long for a batch");
             bs.lines_per_block[d] = static_cast<int32_t>(lpb);
@@ -587,7 +587,7 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
     // workgroups), the medium tiles (two-wave workgroups) and the small
     // tiles (one-wave workgroups), each longest first
     std::vector<std::pair<int64_t, BatchSolveItem>> large_items, medium_items, small_items;
-    for (int pass = 0; pass < kMaxDim; ++pass) {
+    for (int pass = 0; pass < kBatchDim; ++pass) {
         SolveGroup gl{pass, 0, 0, 0, kTileThreads};
         SolveGroup gm{pass, 0, 0, 0, kMediumSolveThreads};
         SolveGroup gs{pass, 0, 0, 0, kSmallSolveThreads};
@@ -598,13 +598,13 @@ int make_solve_work(intp_batch b, hipStream_t stream) {
             const int D = b->dims[static_cast<size_t>(s)];
             if (D <= pass) continue;
             const int a = D - 1 - pass;
-            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + a];
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kBatchDim + a];
             if (ds.lu.p > (ds.lu.cyclic ? 3 : kMaxBatchBand))
                 return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
             const BatchSpline& bs = b->host_specs[static_cast<size_t>(s)];
             const int64_t nlines = bs.total / ds.n;
             int64_t inner = 1;
-            for (int d = a + 1; d < D; ++d) inner *= b->setups[static_cast<size_t>(s) * kMaxDim + d].n;
+            for (int d = a + 1; d < D; ++d) inner *= b->setups[static_cast<size_t>(s) * kBatchDim + d].n;
             const int64_t cost = ds.n * row_cost(ds);
             if (D == 1) {
                 continue;  // batch_line_kernel, beside the passes
@@ -665,7 +665,7 @@ int make_eval_work(intp_batch b, const int64_t* point_offsets, hipStream_t strea
     b->eval_host.clear();
     b->eval_groups.clear();
     const int64_t per = b->tune.eval_threads;
-    for (int D = 1; D <= kMaxDim; ++D)
+    for (int D = 1; D <= kBatchDim; ++D)
         for (int K = 0; K <= kMaxOrder; ++K) {
             EvalGroup g{D, K, static_cast<int64_t>(b->eval_host.size()), 0};
             for (int s = 0; s < b->count; ++s) {
@@ -708,7 +708,7 @@ int prepare_split(intp_batch b, hipStream_t stream) {
     std::vector<double> tab;
     std::vector<std::pair<int32_t, size_t>> at;  // spline, offset of its gf (gb follows)
     for (int32_t s : b->list1d_host) {
-        const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim];
+        const DimSetup& ds = b->setups[static_cast<size_t>(s) * kBatchDim];
         const LineFactors& lu = ds.lu;
         const int BW = lu.p;
         const int64_t n = ds.n, H = n - 2 * BW - 8;
@@ -964,12 +964,12 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
     std::unique_ptr<intp_batch_s> b(new intp_batch_s());
     b->device = device;
     b->count = count;
-    b->setups.resize(static_cast<size_t>(count) * kMaxDim);
+    b->setups.resize(static_cast<size_t>(count) * kBatchDim);
     b->dims.resize(static_cast<size_t>(count));
     b->orders.resize(static_cast<size_t>(count));
     for (int s = 0; s < count; ++s) {
         const intp_grid_desc& g = descs[s];
-        if (g.dim < 1 || g.dim > kMaxDim || g.order < 0 || g.order > kMaxOrder)
+        if (g.dim < 1 || g.dim > kBatchDim || g.order < 0 || g.order > kMaxOrder)
             return fail(INTP_E_UNSUPPORTED, "spline " + std::to_string(s) + ": no kernels for this dimension/order");
         b->dims[static_cast<size_t>(s)] = g.dim;
         b->orders[static_cast<size_t>(s)] = g.order;
@@ -983,7 +983,7 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
             for (int d = 0; d < g.dim; ++d) {
                 const std::string err =
                     setup_dimension(g.order, g.periodic[d] != 0, g.n[d], g.xmin[d], g.xmax[d], g.coords[d],
-                                    b->setups[static_cast<size_t>(s) * kMaxDim + d]);
+                                    b->setups[static_cast<size_t>(s) * kBatchDim + d]);
                 if (!err.empty()) {
                     errors[static_cast<size_t>(s)] = err + " (dimension " + std::to_string(d) + ")";
                     break;
@@ -1028,13 +1028,13 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
     struct Where {
         size_t knots, recip, poly, lower, upper, diag, bottom, right;
     };
-    std::vector<Where> where(static_cast<size_t>(count) * kMaxDim);
+    std::vector<Where> where(static_cast<size_t>(count) * kBatchDim);
     for (int s = 0; s < count; ++s) {
         int64_t total = 1;
         for (int d = 0; d < b->dims[static_cast<size_t>(s)]; ++d) {
-            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + d];
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kBatchDim + d];
             total *= ds.n;
-            Where& w = where[static_cast<size_t>(s) * kMaxDim + d];
+            Where& w = where[static_cast<size_t>(s) * kBatchDim + d];
             w.knots = place(ds.knots);
             w.recip = place(ds.recip);
             w.poly = place(ds.poly);
@@ -1075,10 +1075,10 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
         bs.total = b->offsets[static_cast<size_t>(s) + 1] - bs.offset;
         EvalParams& ep = bs.ep;
         for (int d = 0; d < D; ++d) {
-            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kMaxDim + d];
-            const Where& w = where[static_cast<size_t>(s) * kMaxDim + d];
+            const DimSetup& ds = b->setups[static_cast<size_t>(s) * kBatchDim + d];
+            const Where& w = where[static_cast<size_t>(s) * kBatchDim + d];
             int64_t stride = 1;
-            for (int e = d + 1; e < D; ++e) stride *= b->setups[static_cast<size_t>(s) * kMaxDim + e].n;
+            for (int e = d + 1; e < D; ++e) stride *= b->setups[static_cast<size_t>(s) * kBatchDim + e].n;
             ep.dim[d] = make_dim_params(ds, K, stride, dptr(w.knots, ds.knots.size()), dptr(w.recip, ds.recip.size()),
                                         dptr(w.poly, ds.poly.size()));
             bs.lower[d] = dptr(w.lower, ds.lu.lower.size());
@@ -1100,13 +1100,13 @@ int intp_batch_create(intp_ctx ctx_, int32_t count, const intp_grid_desc* descs,
     // the 1D splines, longest first (they run beside the passes)
     for (int s = 0; s < count; ++s)
         if (b->dims[static_cast<size_t>(s)] == 1) {
-            const LineFactors& lu = b->setups[static_cast<size_t>(s) * kMaxDim].lu;
+            const LineFactors& lu = b->setups[static_cast<size_t>(s) * kBatchDim].lu;
             if (lu.p > (lu.cyclic ? 3 : kMaxBatchBand))
                 return fail(INTP_E_UNSUPPORTED, "band width without a compiled solver");
             b->list1d_host.push_back(s);
         }
     std::stable_sort(b->list1d_host.begin(), b->list1d_host.end(), [&](int32_t x, int32_t y) {
-        return b->setups[static_cast<size_t>(x) * kMaxDim].n > b->setups[static_cast<size_t>(y) * kMaxDim].n;
+        return b->setups[static_cast<size_t>(x) * kBatchDim].n > b->setups[static_cast<size_t>(y) * kBatchDim].n;
     });
     if (!b->list1d_host.empty()) {
         HIP_TRY(device_alloc(reinterpret_cast<void**>(&b->list1d), b->list1d_host.size() * sizeof(int32_t), stream),

@@ -11,20 +11,22 @@
 
 namespace intp_amd {
 
+constexpr int kBatchDim = 3;  // the batched form takes 1D, 2D and 3D splines
+
 // One spline of a batch, in device memory.
 struct BatchSpline {
     EvalParams ep;  // dims, control points (ctrl), value-only output layout
-    const double* lower[kMaxDim];
-    const double* upper[kMaxDim];
-    const double* diag[kMaxDim];
-    const double* bottom[kMaxDim];
-    const double* right[kMaxDim];
+    const double* lower[kBatchDim];
+    const double* upper[kBatchDim];
+    const double* diag[kBatchDim];
+    const double* bottom[kBatchDim];
+    const double* right[kBatchDim];
     int64_t offset;  // first value / control point of the spline in the batch arrays
     int64_t total;   // its control points
-    int64_t shift[kMaxDim];
+    int64_t shift[kBatchDim];
     int32_t dim, order;
-    int32_t lines_per_block[kMaxDim];  // LDS tile of the solve along each axis
-    int32_t band[kMaxDim];             // band width of each axis's LU
+    int32_t lines_per_block[kBatchDim];  // LDS tile of the solve along each axis
+    int32_t band[kBatchDim];             // band width of each axis's LU
     // Split solve of a long 1D line (INTP_BATCH_SPLIT_LINES): nseg segments
     // of the first n - 2*band - 8 rows, and the responses of each segment's
     // rows to the band values entering it, forward (gf) and backward (gb),

@@ -35,7 +35,7 @@ __device__ __forceinline__ uint32_t brick_key(const BinParams& p, const double* 
 
 __device__ __forceinline__ void load_point(const BinParams& p, int64_t i, double* x) {
 #pragma unroll
-    for (int d = 0; d < kMaxDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
+    for (int d = 0; d < kGroupDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
 }
 
 // A record read once, as two 16-byte non-temporal loads.  Streamed
@@ -90,14 +90,14 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
     const int64_t first = chunk * kChunk;
     // every load of the thread is issued before the first digit is formed
     // (a load per LDS atomic leaves one load in flight per thread)
-    double x[kPerThread][kMaxDim];
+    double x[kPerThread][kGroupDim];
     uint32_t key[kPerThread];
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
         const int64_t i = first + j * kThreads + threadIdx.x;
         if constexpr (PASS == 0) {
 #pragma unroll
-            for (int d = 0; d < kMaxDim; ++d) x[j][d] = 0.0;
+            for (int d = 0; d < kGroupDim; ++d) x[j][d] = 0.0;
             if (i < p.npts) load_point(p, i, x[j]);
         } else {
             key[j] = i < p.npts ? p.key_in[i] : 0u;

@@ -1,0 +1,10 @@
+// eval_f32_d6.hip -- evaluation kernels for 6D splines with float tables.
+#include "eval_impl.hpp"
+
+namespace intp_amd {
+
+hipError_t eval_dispatch_f32_d6(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s) {
+    return eval_dispatch<6, float>(order, grad, uniform, p, s);
+}
+
+}  // namespace intp_amd

@@ -360,6 +360,55 @@ __device__ __forceinline__ void shift_down(DimW<K, Off, W>& d) {
     }
 }
 
+// 4 to 6 dimensions: the outer D-1 dimensions run as one rolled loop
+// over their (K+1)^(D-1) index combinations (a counter with the last outer
+// dimension fastest), each contracting one stencil row along the last
+// dimension.  The weights are indexed at run time, so these kernels keep
+// them in scratch memory.  GRAD: res = value and the D first partials
+// (dw along one dimension, w along the others); otherwise res[0] only.
+template <int D, int K, bool GRAD, bool LASTC, typename T, typename Off>
+__device__ void contract_nd(const T* __restrict__ c, DimW<K, Off, T> (&dd)[D], T (&res)[D + 1]) {
+    int a[D - 1];
+#pragma unroll
+    for (int d = 0; d < D - 1; ++d) a[d] = 0;
+#pragma unroll
+    for (int s = 0; s <= D; ++s) res[s] = T(0);
+    for (;;) {
+        Off rowoff = 0;
+        T wv = T(1);
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d) {
+            rowoff += dd[d].off[a[d]];
+            wv *= dd[d].w[a[d]];
+        }
+        T cv[K + 1];
+        load_row<K, LASTC, T>(c, rowoff, dd[D - 1], cv);
+        T sB = T(0), sD = T(0);
+#pragma unroll
+        for (int e = 0; e <= K; ++e) {
+            sB += dd[D - 1].w[e] * cv[e];
+            if constexpr (GRAD) sD += dd[D - 1].dw[e] * cv[e];
+        }
+        res[0] += wv * sB;
+        if constexpr (GRAD) {
+#pragma unroll
+            for (int d = 0; d < D - 1; ++d) {
+                T wd = T(1);
+#pragma unroll
+                for (int e = 0; e < D - 1; ++e) wd *= e == d ? dd[e].dw[a[e]] : dd[e].w[a[e]];
+                res[d + 1] += wd * sB;
+            }
+            res[D] += wv * sD;
+        }
+        int d = D - 2;
+        for (; d >= 0; --d) {
+            if (++a[d] <= K) break;
+            a[d] = 0;
+        }
+        if (d < 0) break;
+    }
+}
+
 template <int D, int K, bool LASTC, typename T, typename Off>
 __device__ __forceinline__ void contract_grad(const T* __restrict__ c, DimW<K, Off, T> (&dd)[D], T (&res)[D + 1]) {
     if constexpr (D == 1) {
@@ -392,6 +441,8 @@ __device__ __forceinline__ void contract_grad(const T* __restrict__ c, DimW<K, O
         res[0] = v;
         res[1] = g0;
         res[2] = g1;
+    } else if constexpr (D >= 4) {
+        contract_nd<D, K, true, LASTC>(c, dd, res);
     } else {
         T v = T(0), g0 = T(0), g1 = T(0), g2 = T(0);
         constexpr int kOuterUnroll = K <= INTP_OUTER_UNROLL_MAX_K ? K + 1 : 1;
@@ -443,6 +494,10 @@ __device__ __forceinline__ T contract_single(const T* __restrict__ c, DimW<K, Of
             for (int b = 0; b <= K; ++b) sum += dd[1].w[b] * cv[b];
             v += dd[0].w[a] * sum;
         }
+    } else if constexpr (D >= 4) {
+        T res[D + 1];
+        contract_nd<D, K, false, LASTC>(c, dd, res);
+        v = res[0];
     } else {
         constexpr int kOuterUnroll = K <= INTP_OUTER_UNROLL_MAX_K ? K + 1 : 1;
 #pragma unroll kOuterUnroll
@@ -939,7 +994,7 @@ hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStrea
     if (!grad && !kSingle) return hipErrorInvalidValue;
     if (uniform && p.items) {
         // brick items of the one-pass grouping (2D and 3D grids)
-        if constexpr (D >= 2) {
+        if constexpr (D >= 2 && D <= kGroupDim) {
             if (!p.item_count || p.cs_dim != 1 || p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
                 return hipErrorInvalidValue;
             const size_t lds = static_cast<size_t>(p.patch_volume) * sizeof(T);
@@ -950,11 +1005,13 @@ hipError_t launch_variant(bool grad, bool uniform, const EvalParams& p, hipStrea
     }
     if (uniform && p.patch_volume > 0) {
         // the patch kernels read the grouping passes' records
-        if (D < 2 || p.patch_volume > patch_max<T>() || !p.perm || !p.key || p.cs_dim != 1 ||
-            p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
-            return hipErrorInvalidValue;
-        if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC, T>>(p, s);
-        if constexpr (kSingle) return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
+        if constexpr (D >= 2 && D <= kGroupDim) {
+            if (p.patch_volume > patch_max<T>() || !p.perm || !p.key || p.cs_dim != 1 ||
+                p.cs_pt != static_cast<int64_t>(sizeof(BinRecord) / sizeof(double)))
+                return hipErrorInvalidValue;
+            if (grad) return launch_chunks<eval_patch_kernel<D, K, true, LASTC, T>>(p, s);
+            if constexpr (kSingle) return launch_chunks<eval_patch_kernel<D, K, false, LASTC, T>>(p, s);
+        }
         return hipErrorInvalidValue;
     }
     if (uniform) {

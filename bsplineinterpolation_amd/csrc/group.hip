@@ -30,7 +30,7 @@ constexpr int kBatch = 4;  // points in flight per thread
 
 __device__ __forceinline__ void load_coords(const GroupParams& p, int64_t i, double* x) {
 #pragma unroll
-    for (int d = 0; d < kMaxDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
+    for (int d = 0; d < kGroupDim; ++d) x[d] = d < p.dim_count ? p.coords[d * p.cs_dim + i * p.cs_pt] : 0.0;
 }
 
 __global__ __launch_bounds__(kGroupThreads) void group_hist_kernel(GroupParams p) {
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_hist_kernel(GroupParams p
     const int64_t lo = static_cast<int64_t>(blockIdx.x) * p.per_block;
     const int64_t hi = min(p.npts, lo + p.per_block);
     for (int64_t base = lo; base < hi; base += kGroupThreads * kBatch) {
-        double x[kBatch][kMaxDim];
+        double x[kBatch][kGroupDim];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const int64_t i = base + j * kGroupThreads + threadIdx.x;

@@ -14,13 +14,16 @@
 //  * the products and sums in the reference's own order, so that values,
 //    derivatives and proxy results equal the reference's bit for bit.
 // Dimension and order are template arguments (a table of instantiations):
-// the stencil loops unroll and no index is split by division.
+// the stencil loops unroll and no index is split by division.  Splines of
+// 4 to 6 dimensions take a runtime-dimension form of the same loops, with
+// the same operation order.
 #include "host_eval.hpp"
 
 #include <array>
 #include <cmath>
 #include <cstddef>
 #include <utility>
+#include <vector>
 
 namespace intp_amd {
 
@@ -267,18 +270,170 @@ const std::array<std::array<ApplyFn, kMaxK + 1>, 3> kApplyFns = {
     apply_fns<2>(std::make_integer_sequence<int, kMaxK + 1>{}),
     apply_fns<3>(std::make_integer_sequence<int, kMaxK + 1>{})};
 
+// ---- 4 to 6 dimensions: the loops above with the dimension at run time ----
+
+// The (K+1)^D stencil as a counter a[0..D-1] with dimension `fastest`
+// varying fastest (0: the reference's cell order; D-1: C order).
+template <int K>
+struct StencilCounter {
+    int a[kHostMaxDim] = {};
+    int dim;
+    bool next_dim0_fastest() {
+        for (int d = 0; d < dim; ++d) {
+            if (++a[d] <= K) return true;
+            a[d] = 0;
+        }
+        return false;
+    }
+    bool next_c_order() {
+        for (int d = dim - 1; d >= 0; --d) {
+            if (++a[d] <= K) return true;
+            a[d] = 0;
+        }
+        return false;
+    }
+};
+
+template <int K>
+void all_terms_nd(const HostSpline& s, const double* x, const int32_t* r, double (&b)[kHostMaxDim][K + 1],
+                  int64_t (&off)[kHostMaxDim][K + 1], int64_t (&mu)[kHostMaxDim]) {
+    int64_t stride = 1;
+    for (int d = s.dim - 1; d >= 0; --d) {
+        mu[d] = dimension_terms<K>(*s.dims[d], stride, x[d], r ? r[d] : 0, b[d], off[d]);
+        stride *= s.dims[d]->n;
+    }
+}
+
+// value_at for D >= 4: terms ((c * b_0) * b_1) ... in cell order
+template <int K>
+double value_at_nd(const HostSpline& s, const double* x) {
+    double b[kHostMaxDim][K + 1];
+    int64_t off[kHostMaxDim][K + 1], mu[kHostMaxDim];
+    all_terms_nd<K>(s, x, nullptr, b, off, mu);
+    StencilCounter<K> it;
+    it.dim = s.dim;
+    double v = 0.0;
+    do {
+        int64_t o = 0;
+        for (int d = 0; d < s.dim; ++d) o += off[d][it.a[d]];
+        double coef = s.ctrl[o];
+        for (int d = 0; d < s.dim; ++d) coef *= b[d][it.a[d]];
+        v += coef;
+    } while (it.next_dim0_fastest());
+    return v;
+}
+
+// derivative_at for D >= 4: C-order gather, differences along each
+// differentiated dimension, then the C-order sum of (b_0 b_1 ...) * c
+template <int K>
+double derivative_at_nd(const HostSpline& s, const double* x, const int32_t* r) {
+    constexpr int W = K + 1;
+    const int D = s.dim;
+    for (int d = 0; d < D; ++d)
+        if (r[d] > K) return 0.0;
+    double b[kHostMaxDim][K + 1];
+    int64_t off[kHostMaxDim][K + 1], mu[kHostMaxDim];
+    all_terms_nd<K>(s, x, r, b, off, mu);
+    int64_t total = 1;
+    for (int d = 0; d < D; ++d) total *= W;
+    thread_local std::vector<double> buf;
+    buf.resize(static_cast<size_t>(total));
+    double* c = buf.data();
+    {
+        StencilCounter<K> it;
+        it.dim = D;
+        int64_t l = 0;
+        do {
+            int64_t o = 0;
+            for (int d = 0; d < D; ++d) o += off[d][it.a[d]];
+            c[l++] = s.ctrl[o];
+        } while (it.next_c_order());
+    }
+    for (int d = 0; d < D; ++d) {
+        if (r[d] == 0) continue;
+        const double* t = s.dims[d]->knots.data();
+        int64_t lstride = 1;
+        for (int e = d + 1; e < D; ++e) lstride *= W;
+        const int64_t fibers = total / W;
+        for (int64_t f = 0; f < fibers; ++f) {
+            const int64_t outer = f / lstride, inner = f % lstride;
+            double* v = c + outer * lstride * W + inner;
+            for (int kk = K; kk > K - r[d]; --kk)
+                for (int j = kk; j > 0; --j) {
+                    const int at = K + j - kk;
+                    v[at * lstride] = static_cast<double>(kk) * (v[at * lstride] - v[(at - 1) * lstride]) /
+                                      (t[mu[d] + j] - t[mu[d] + j - kk]);
+                }
+        }
+    }
+    StencilCounter<K> it;
+    it.dim = D;
+    double v = 0.0;
+    int64_t l = 0;
+    do {
+        double coef = b[0][it.a[0]];
+        for (int d = 1; d < D; ++d) coef *= b[d][it.a[d]];
+        v += coef * c[l++];
+    } while (it.next_c_order());
+    return v;
+}
+
+template <int K>
+double eval_point_nd(const HostSpline& s, const double* x, const int32_t* r) {
+    for (int d = 0; d < s.dim; ++d)
+        if (r[d] != 0) return derivative_at_nd<K>(s, x, r);
+    return value_at_nd<K>(s, x);
+}
+
+// apply_weights for D >= 4 (value_at_nd's order)
+template <int K>
+double apply_weights_nd(const HostSpline& s, const int32_t* first, const double* w) {
+    constexpr int W = K + 1;
+    int64_t off[kHostMaxDim][W];
+    int64_t stride = 1;
+    for (int d = s.dim - 1; d >= 0; --d) {
+        for (int m = 0; m < W; ++m) off[d][m] = stencil_index(*s.dims[d], first[d], m) * stride;
+        stride *= s.dims[d]->n;
+    }
+    StencilCounter<K> it;
+    it.dim = s.dim;
+    double v = 0.0;
+    do {
+        int64_t o = 0;
+        for (int d = 0; d < s.dim; ++d) o += off[d][it.a[d]];
+        double coef = s.ctrl[o];
+        for (int d = 0; d < s.dim; ++d) coef *= w[d * W + it.a[d]];
+        v += coef;
+    } while (it.next_dim0_fastest());
+    return v;
+}
+
+template <int... K>
+constexpr std::array<PointFn, sizeof...(K)> point_fns_nd(std::integer_sequence<int, K...>) {
+    return {&eval_point_nd<K>...};
+}
+template <int... K>
+constexpr std::array<ApplyFn, sizeof...(K)> apply_fns_nd(std::integer_sequence<int, K...>) {
+    return {&apply_weights_nd<K>...};
+}
+const std::array<PointFn, kMaxK + 1> kPointFnsNd = point_fns_nd(std::make_integer_sequence<int, kMaxK + 1>{});
+const std::array<ApplyFn, kMaxK + 1> kApplyFnsNd = apply_fns_nd(std::make_integer_sequence<int, kMaxK + 1>{});
+
+PointFn point_fn(int dim, int order) { return dim <= 3 ? kPointFns[dim - 1][order] : kPointFnsNd[order]; }
+ApplyFn apply_fn(int dim, int order) { return dim <= 3 ? kApplyFns[dim - 1][order] : kApplyFnsNd[order]; }
+
 }  // namespace
 
 int64_t host_eval_points(const HostSpline& s, int64_t npts, const double* coords, int64_t cs_dim, int64_t cs_pt,
                          int nout, const int32_t* orders, double* out, int64_t os_out, int64_t os_pt, bool bounds) {
     const int D = s.dim;
+    const PointFn fn = point_fn(D, s.order);
     for (int64_t i = 0; i < npts; ++i) {
-        double x[3];
+        double x[kHostMaxDim];
         for (int d = 0; d < D; ++d) x[d] = coords[d * cs_dim + i * cs_pt];
         if (bounds)
             for (int d = 0; d < D; ++d)
                 if (outside(*s.dims[d], x[d])) return i;
-        const PointFn fn = kPointFns[D - 1][s.order];
         for (int o = 0; o < nout; ++o) out[o * os_out + i * os_pt] = fn(s, x, orders + o * D);
     }
     return -1;
@@ -302,7 +457,7 @@ int64_t host_proxy_weights(const HostSpline& s, int64_t npts, const double* coor
 void host_proxy_apply(const HostSpline& s, int64_t npts, const int32_t* first, const double* weights, double* out,
                       int64_t os_pt) {
     const int D = s.dim, w = s.order + 1;
-    const ApplyFn fn = kApplyFns[D - 1][s.order];
+    const ApplyFn fn = apply_fn(D, s.order);
     for (int64_t i = 0; i < npts; ++i) out[i * os_pt] = fn(s, first + i * D, weights + i * D * w);
 }
 

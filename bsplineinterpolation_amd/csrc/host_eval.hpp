@@ -11,10 +11,12 @@
 
 namespace intp_amd {
 
+constexpr int kHostMaxDim = 6;
+
 // One spline as the host evaluator sees it: per-dimension setups (knots,
 // range, periodicity), the order and the C-order control points.
 struct HostSpline {
-    const DimSetup* dims[3];
+    const DimSetup* dims[kHostMaxDim];
     int dim = 0;
     int order = 0;
     const double* ctrl = nullptr;

@@ -9,7 +9,8 @@
 
 namespace intp_amd {
 
-constexpr int kMaxDim = 3;
+constexpr int kMaxDim = 6;    // dimensions of a spline
+constexpr int kGroupDim = 3;  // dimensions the grouping by brick handles (2 and 3)
 constexpr int kMaxOrder = 7;
 
 // One dimension of a spline as the evaluation kernels see it.
@@ -40,8 +41,8 @@ struct DimParams {
 constexpr int kPatchBytes = 24576;
 
 struct BrickGrid {
-    int32_t brick_shift[kMaxDim];
-    int32_t nbrick[kMaxDim];
+    int32_t brick_shift[kGroupDim];
+    int32_t nbrick[kGroupDim];
 };
 
 // A run of grouped points that share one brick (one-pass grouping,
@@ -91,13 +92,13 @@ struct EvalParams {
 
 // A query point carried through the grouping passes (32 bytes).
 struct BinRecord {
-    double x[kMaxDim];
+    double x[kGroupDim];
     uint32_t idx;  // position in the caller's arrays
     uint32_t key;  // brick id
 };
 
 struct BinParams {
-    DimParams dim[kMaxDim];  // the grid (span search), uniform dimensions only
+    DimParams dim[kGroupDim];  // the grid (span search), uniform dimensions only
     BrickGrid bricks;
     int32_t dim_count;
     int32_t order;
@@ -205,13 +206,20 @@ __device__ __forceinline__ int64_t xcd_chunk_index() {
 hipError_t launch_eval(int dim, int order, bool grad, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
 hipError_t launch_eval_grad(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
 hipError_t launch_eval_single(int dim, int order, bool uniform, bool f32, const EvalParams& p, hipStream_t s);
-// per table type and dimension (eval_f64_d*.hip, eval_f32_d*.hip)
+// per table type and dimension (eval_f64_d*.hip, eval_f32_d*.hip; 4 to 6
+// dimensions contract the stencil in a rolled loop)
 hipError_t eval_dispatch_f64_d1(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t eval_dispatch_f64_d2(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t eval_dispatch_f64_d3(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t eval_dispatch_f32_d1(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t eval_dispatch_f32_d2(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t eval_dispatch_f32_d3(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f64_d4(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f64_d5(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f64_d6(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d4(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d5(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
+hipError_t eval_dispatch_f32_d6(int order, bool grad, bool uniform, const EvalParams& p, hipStream_t s);
 hipError_t launch_solve(int bw, bool cyclic, SolveTypes types, const SolveParams& p, hipStream_t s);
 // Lines of length n that one LDS-tiled solve workgroup holds on the
 // current device (a power of two up to 64) for compute elements of
@@ -231,7 +239,7 @@ constexpr int kBrickItemPoints = kBrickThreads * kBrickPointsPerThread;
 constexpr int kBrickPatchBytes = 64 * 1024 - 256;  // dynamic LDS of the brick kernel
 
 struct GroupParams {
-    DimParams dim[kMaxDim];
+    DimParams dim[kGroupDim];
     BrickGrid bricks;
     int32_t dim_count;
     int32_t order;
@@ -330,7 +338,10 @@ struct alignas(8) Cell1 {
 struct alignas(16) Cell3 {
     float c, e0, e1, e2;  // 2D tables leave e2 at 0
 };
+constexpr int kCellMaxDim = 3;  // refined builds of more dimensions keep no gradient table
 inline int cell_bytes(int dim) { return dim == 1 ? static_cast<int>(sizeof(Cell1)) : static_cast<int>(sizeof(Cell3)); }
+// gradient-table bytes per control point of a refined build (0: none)
+inline int gradient_table_bytes(int dim) { return dim <= kCellMaxDim ? cell_bytes(dim) : 0; }
 
 struct CellParams {
     const float* c0;     // FP32 control points

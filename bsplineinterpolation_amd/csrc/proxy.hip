@@ -178,6 +178,9 @@ hipError_t launch_proxy_create(int dim, int order, bool uniform, const ProxyPara
         case 1: return create_d<1>(order, uniform, p, s);
         case 2: return create_d<2>(order, uniform, p, s);
         case 3: return create_d<3>(order, uniform, p, s);
+        case 4: return create_d<4>(order, uniform, p, s);
+        case 5: return create_d<5>(order, uniform, p, s);
+        case 6: return create_d<6>(order, uniform, p, s);
         default: return hipErrorNotSupported;
     }
 }
@@ -190,6 +193,12 @@ hipError_t launch_proxy_apply(int dim, int order, bool uniform, bool f32, const 
         case 5: return apply_d<2, float>(order, uniform, p, s);
         case 6: return apply_d<3, double>(order, uniform, p, s);
         case 7: return apply_d<3, float>(order, uniform, p, s);
+        case 8: return apply_d<4, double>(order, uniform, p, s);
+        case 9: return apply_d<4, float>(order, uniform, p, s);
+        case 10: return apply_d<5, double>(order, uniform, p, s);
+        case 11: return apply_d<5, float>(order, uniform, p, s);
+        case 12: return apply_d<6, double>(order, uniform, p, s);
+        case 13: return apply_d<6, float>(order, uniform, p, s);
         default: return hipErrorNotSupported;
     }
 }

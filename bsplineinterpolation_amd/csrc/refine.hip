@@ -261,7 +261,7 @@ hipError_t launch_band_matvec(const MatvecParams& p_in, bool in_f32, bool out_f3
 }
 
 hipError_t launch_refined_cells(const CellParams& p_in, hipStream_t s) {
-    if (p_in.total >= (int64_t{1} << 32) || p_in.dim < 1 || p_in.dim > kMaxDim) return hipErrorInvalidValue;
+    if (p_in.total >= (int64_t{1} << 32) || p_in.dim < 1 || p_in.dim > kCellMaxDim) return hipErrorInvalidValue;
     CellParams p = p_in;
     int64_t total = 1;
     for (int d = 0; d < p.dim; ++d) {

@@ -89,7 +89,7 @@ struct Resident {
 int64_t resident_bytes_of(const intp_grid_desc& d, int precision) {
     int64_t n = 1;
     for (int i = 0; i < d.dim; ++i) n *= d.n[i];
-    return n * (precision == INTP_PREC_F32_REFINED ? 4 + cell_bytes(d.dim) : 8);
+    return n * (precision == INTP_PREC_F32_REFINED ? 4 + gradient_table_bytes(d.dim) : 8);
 }
 
 int64_t build_footprint(const intp_grid_desc& d, int precision) {
@@ -99,17 +99,19 @@ int64_t build_footprint(const intp_grid_desc& d, int precision) {
 }
 
 // Algorithmic HBM bytes of a build (2 D N s per solve; refined: S+1 FP32
-// solves, S FP64 residuals with the subtraction of f, S-1 FP32 updates
-// added on store, the gradient table from c (and the last correction),
-// and the last update).
+// solves, S FP64 residuals with the subtraction of f, the FP32 updates
+// added on store, and with a gradient table (up to 3 dimensions) the
+// table from c and the last correction, and the last update).
 double build_bytes(const intp_grid_desc& d, int precision, int sweeps) {
     double n = 1;
     for (int i = 0; i < d.dim; ++i) n *= static_cast<double>(d.n[i]);
     const double D = d.dim;
     if (precision != INTP_PREC_F32_REFINED) return 2 * D * n * 8;
     const double S = sweeps;
-    const double cells = n * ((S > 0 ? 8 : 4) + cell_bytes(d.dim)) + (S > 0 ? 12 * n : 0);
-    return (S + 1) * 2 * D * n * 4 + S * (2 * D * n * 8 + 8 * n) + (S > 0 ? S - 1 : 0) * n * 4 + cells;
+    const double base = (S + 1) * 2 * D * n * 4 + S * (2 * D * n * 8 + 8 * n);
+    const int cb = gradient_table_bytes(d.dim);
+    if (cb == 0) return base + S * n * 4;
+    return base + (S > 0 ? S - 1 : 0) * n * 4 + n * ((S > 0 ? 8 : 4) + cb) + (S > 0 ? 12 * n : 0);
 }
 
 }  // namespace

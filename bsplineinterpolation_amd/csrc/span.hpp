@@ -74,7 +74,7 @@ __device__ __forceinline__ uint32_t brick_key_of(const DimParams* dims, int ndim
 #pragma clang fp contract(off)
     uint32_t key = 0;
 #pragma unroll
-    for (int d = 0; d < kMaxDim; ++d) {
+    for (int d = 0; d < kGroupDim; ++d) {
         if (d >= ndim) break;
         const DimParams& dp = dims[d];
         double xd = x[d];

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define INTP_MAX_DIM 3   /* dimensions with compiled kernels            */
+#define INTP_MAX_DIM 6   /* dimensions with compiled kernels            */
 #define INTP_MAX_ORDER 7 /* spline orders with compiled kernels         */
 
 typedef enum {

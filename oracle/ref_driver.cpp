@@ -142,6 +142,14 @@ RefBase* build_k(int dim, const int* per, const int64_t* n, const double* lo,
         if (nu0 && nu1 && nu2) return new RefFn<3, K>(F3(make_per<3>(per), m, cr(0), cr(1), cr(2)));
         throw std::invalid_argument("mixed uniform/non-uniform 3D not instantiated");
     }
+    if (dim == 4) {
+        // uniform ranges only (the 4D pinning cases)
+        if (xc && (xc[0] || xc[1] || xc[2] || xc[3]))
+            throw std::invalid_argument("non-uniform 4D not instantiated");
+        auto m = make_mesh<4>(n, f);
+        using F4 = intp::InterpolationFunction<double, 4, K>;
+        return new RefFn<4, K>(F4(make_per<4>(per), m, ur(0), ur(1), ur(2), ur(3)));
+    }
     throw std::invalid_argument("dim not instantiated");
 }
 

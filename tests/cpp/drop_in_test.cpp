@@ -237,12 +237,47 @@ void value_type_checks() {
     (void)proxy;
 }
 
+// 4D: InterpolationFunction<double, 4, 3> of a separable product against
+// the products of the 1D splines of its factors (tensor-product
+// interpolation of a rank-one mesh is rank one), value and one partial.
+void high_dim_checks() {
+    const std::array<std::size_t, 4> n{8, 7, 6, 9};
+    const std::array<bool, 4> per{true, false, false, true};
+    std::array<std::vector<double>, 4> g;
+    for (std::size_t d = 0; d < 4; ++d)
+        for (std::size_t i = 0; i < n[d]; ++i) g[d].push_back(std::cos(0.7 * static_cast<double>(i) + d) + 0.1 * d);
+    intp::Mesh<double, 4> m(n[0], n[1], n[2], n[3]);
+    for (std::size_t a = 0; a < n[0]; ++a)
+        for (std::size_t b = 0; b < n[1]; ++b)
+            for (std::size_t c = 0; c < n[2]; ++c)
+                for (std::size_t e = 0; e < n[3]; ++e) m(a, b, c, e) = g[0][a] * g[1][b] * g[2][c] * g[3][e];
+    const auto r = std::make_pair(0.0, 1.0);
+    const intp::InterpolationFunction<double, 4, 3> f4(per, m, r, r, r, r);
+    const std::array<intp::InterpolationFunction1D<3>, 4> f1{
+        intp::InterpolationFunction1D<3>(r, intp::util::get_range(g[0]), per[0]),
+        intp::InterpolationFunction1D<3>(r, intp::util::get_range(g[1]), per[1]),
+        intp::InterpolationFunction1D<3>(r, intp::util::get_range(g[2]), per[2]),
+        intp::InterpolationFunction1D<3>(r, intp::util::get_range(g[3]), per[3])};
+    std::vector<double> got, want, got_d, want_d;
+    for (int i = 0; i < 100; ++i) {
+        const std::array<double, 4> x{(i * 37 % 100) / 99.0, (i * 53 % 100) / 99.0, (i * 71 % 100) / 99.0, i / 99.0};
+        got.push_back(f4(x));
+        want.push_back(f1[0](x[0]) * f1[1](x[1]) * f1[2](x[2]) * f1[3](x[3]));
+        got_d.push_back(f4.derivative(x, std::array<std::size_t, 4>{0, 1, 0, 0}));
+        want_d.push_back(f1[0](x[0]) * f1[1].derivative(std::make_pair(x[1], std::size_t{1})) * f1[2](x[2]) *
+                         f1[3](x[3]));
+    }
+    report("4D value = product of 1D splines", got.data(), want.data(), got.size(), 1e-12);
+    report("4D d/dx1 = product with 1D derivative", got_d.data(), want_d.data(), got_d.size(), 1e-12);
+}
+
 }  // namespace
 
 int main() {
 #include "mma_cases.inc"
     surface_checks();
     value_type_checks();
+    high_dim_checks();
     std::printf("%d of %d checks failed\n", failures, checks);
     return failures == 0 ? 0 : 1;
 }

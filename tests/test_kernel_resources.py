@@ -22,6 +22,16 @@ LLVM = "/opt/rocm/llvm/bin"
 # kernels allowed to spill, and their scratch bound (bytes per lane): the
 # one-line 1D solve runs at a 128-VGPR bound for four waves per SIMD
 ALLOWED_SCRATCH = {"batch_line_kernel": 128}
+# evaluation kernels of 4 to 6 dimensions walk the outer dimensions of the
+# stencil in a rolled loop and index their weights at run time: they keep
+# the weights in scratch (at most this many bytes per lane)
+ND_KERNELS = ("eval_general_kernel", "eval_uniform_kernel", "proxy_apply_kernel")
+ND_SCRATCH = 2048
+
+
+def _dimension(mangled):
+    m = re.search(r"GLOBAL__N_1\d+[A-Za-z_]+?ILi(\d+)E", mangled)
+    return int(m.group(1)) if m else 0
 
 
 def _kernel_metadata():
@@ -70,6 +80,8 @@ def test_kernels_use_no_scratch():
         scratch = md.get("private_segment_fixed_size", 0)
         short = _short(k)
         limit = next((v for a, v in ALLOWED_SCRATCH.items() if short.endswith(a)), 0)
+        if short.endswith(ND_KERNELS) and _dimension(k) >= 4:
+            limit = ND_SCRATCH
         if scratch > limit:
             bad.append((short, scratch))
     assert not bad, bad

@@ -325,3 +325,34 @@ def test_oracle_derivative_above_order_is_zero(orc):
     f = np.random.default_rng(1).standard_normal((6, 6))
     s = orc.OracleSpline(2, f, ranges=[(0, 1), (0, 1)])
     assert np.all(s.eval([[0.3, 0.4]], [[3, 0], [0, 5]]) == 0.0)
+
+
+@pytest.mark.parametrize("D,k", [(4, 3), (5, 2), (5, 3), (6, 1), (6, 3)])
+def test_oracle_separable_matches_1d_products(orc, D, k):
+    """Dimensions without an instantiated reference (5, 6; 4 also against
+    the compiled reference): interpolating a separable product
+    f(x) = prod_d g_d(x_d) on a tensor grid gives the product of the 1D
+    interpolants of the g_d (the tensor-product solve of a rank-one right-
+    hand side is rank one), and a partial along d replaces factor d by its
+    derivative.  The 1D oracle is pinned against the reference, so a wrong
+    axis, stride or periodic shift in the D-dimensional oracle fails here."""
+    rng = np.random.default_rng(10 * D + k)
+    n = [int(rng.integers(max(k + 2, 5), 9)) for _ in range(D)]
+    per = [bool(d % 2 == 0) for d in range(D)]
+    ranges = [(float(-1 - d * 0.1), float(1 + d * 0.2)) for d in range(D)]
+    gs = [rng.standard_normal(n[d]) for d in range(D)]
+    f = gs[0]
+    for d in range(1, D):
+        f = np.multiply.outer(f, gs[d])
+    s = orc.OracleSpline(k, f, periodic=per, ranges=ranges)
+    ones = [orc.OracleSpline(k, gs[d], periodic=[per[d]], ranges=[ranges[d]]) for d in range(D)]
+    q = np.stack([rng.uniform(ranges[d][0], ranges[d][1], 60) for d in range(D)], axis=1)
+    orders = [[0] * D] + [list(e) for e in np.eye(D, dtype=int)]
+    got = s.eval(q, orders)
+    vals = [ones[d].eval(q[:, [d]], [[0]])[0] for d in range(D)]
+    ders = [ones[d].eval(q[:, [d]], [[1]])[0] for d in range(D)]
+    want = [np.prod(vals, axis=0)]
+    for d in range(D):
+        want.append(np.prod([ders[e] if e == d else vals[e] for e in range(D)], axis=0))
+    for i in range(len(orders)):
+        assert rel_err(got[i], want[i]) < 1e-12, (i, rel_err(got[i], want[i]))

@@ -31,6 +31,11 @@ def _configs():
         for per in itertools.product([False, True], repeat=D):
             n = [int(rng.integers(k + 3, 12)) for _ in range(D)]
             out.append((D, k, list(per), n, True))
+    # 4D (uniform grids): the oracle's loops over any dimension count
+    for k in range(0, 6):
+        for per in ([False] * 4, [True, False, True, False], [True] * 4):
+            n = [int(rng.integers(max(k + 2, 5), 8)) for _ in range(4)]
+            out.append((4, k, per, n, False))
     return out
 
 
