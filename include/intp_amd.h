@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define INTP_MAX_DIM 6   /* dimensions with compiled kernels            */
+#define INTP_MAX_DIM 6   /* dimensions with compiled kernels (batches: 3) */
 #define INTP_MAX_ORDER 7 /* spline orders with compiled kernels         */
 
 typedef enum {
@@ -263,7 +263,7 @@ int intp_proxy_destroy(intp_proxy p);
 /* ---- batches: many independent splines built and evaluated together -- */
 /* No single reference counterpart: a batch is N independent
  * InterpolationFunction<double, D_s, O_s> objects (Interpolation.hpp:92-107)
- * of mixed dimension, order, size and periodicity, each built and evaluated
+ * of mixed dimension (1 to 3), order, size and periodicity, each built and evaluated
  * with the single-spline arithmetic (its control points equal
  * intp_build's), but by a fixed set of persistent launches over all of
  * them.  Layouts:
