@@ -607,6 +607,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the refined-FP32 form measurement")
     ap.add_argument("--only-forms", action="store_true",
                     help="measure only the refined-FP32 form (profiling runs); prints its object")
+    ap.add_argument("--only-refined-headline", action="store_true",
+                    help="measure only the refined-FP32 form of the headline step, beside the FP64 step (profiling "
+                         "runs); prints its object")
     ap.add_argument("--only-serving", action="store_true", help="measure only the serving loop; prints its object")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds of arrivals per serving rate")
     ap.add_argument("--serve-burst", type=int, default=1500, help="requests of the saturation burst")
@@ -642,6 +645,28 @@ def main():
         return
     if args.only_forms:
         print(json.dumps(measure_refined_form(amd, torch, args.steps, args.warmup, n, nq)))
+        return
+    if args.only_refined_headline:
+        w = datagen.workload_3d_256_cubic(n=n, nq=nq)
+        dev = torch.device("cuda:0")
+        f_d = torch.from_numpy(w["f"]).to(dev)
+        q_d = torch.from_numpy(np.ascontiguousarray(w["queries"].T)).to(dev)
+        out64 = torch.empty((nq, 4), dtype=torch.float64, device=dev)
+
+        def step64():
+            s = amd.InterpolationFunction(3, f_d, periodic=w["periodic"], ranges=w["ranges"])
+            s.eval(q_d, w["orders"], out=out64.t())
+            return s
+
+        for _ in range(args.warmup):
+            step64()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step64()
+        torch.cuda.synchronize()
+        f64_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        print(json.dumps(measure_refined_headline(amd, torch, args.steps, args.warmup, w, f_d, q_d, out64, f64_ms)))
         return
     print("bench: generating inputs (%d^3 mesh, %d queries)" % (n, nq), file=sys.stderr, flush=True)
     w = datagen.workload_3d_256_cubic(n=n, nq=nq)

@@ -272,6 +272,44 @@ __device__ __forceinline__ void substitute(P* __restrict__ x, int64_t step, int6
     backward_range<BW, CYC, C, P, B>(x, step, n, 0, n, sp, st, z);
 }
 
+// A launch of a few long lines (a long 1D spline, say) substitutes each
+// line with one lane whose factor loads run one batch of rows ahead: from
+// HBM every batch would wait a full memory latency.  When the launch has
+// at most kWarmLines lines and the factors fit kWarmBytes, every thread of
+// the workgroup first reads the factor arrays once (one 4-byte word per
+// 64 bytes), so that the substitution finds them in L2.  The arithmetic is
+// unchanged.
+constexpr int64_t kWarmLines = 8;
+constexpr int64_t kWarmBytes = int64_t{2} << 20;
+
+template <int BW, bool CYC, typename C, int NT>
+__device__ __forceinline__ void warm_factors(const SolveParams& sp, int64_t n) {
+    if (sp.nlines > kWarmLines) return;
+    const int64_t per = static_cast<int64_t>(sizeof(C)) * n;
+    const int64_t bytes = per * (2 * BW + 1) + (CYC ? per * 2 * BW : 0);
+    if (bytes > kWarmBytes) return;
+    uint32_t acc = 0;
+    auto touch = [&](const void* p, int64_t nbytes) {
+        const uint32_t* w = static_cast<const uint32_t*>(p);
+        const int64_t lines = nbytes / 64;
+        for (int64_t i = threadIdx.x; i < lines; i += NT * 4) {
+            uint32_t v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = i + u * NT < lines ? w[(i + u * NT) * 16] : 0u;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc ^= v[u];
+        }
+    };
+    touch(sp.lower, per * BW);
+    touch(sp.upper, per * BW);
+    touch(sp.diag, per);
+    if constexpr (CYC) {
+        touch(sp.bottom, per * BW);
+        touch(sp.right, per * BW);
+    }
+    asm volatile("" ::"v"(acc));  // the loads' results are used
+}
+
 // Fallback: one lane per line, substituting in HBM (storage T, compute C;
 // with T narrower than C the forward results are rounded to T between
 // the two sweeps).
@@ -283,7 +321,9 @@ __host__ __device__ __forceinline__ int64_t tile_stride(int64_t n) { return n | 
 // One workgroup: lines [blockIdx.x * lpb, + lpb) staged through LDS in
 // the compute type C; the table holds T.  With sp.accum set, the solved
 // lines are added into accum instead of being stored to x.
-template <int BW, bool CYC, typename T, typename C, int B = batch_rows<BW, CYC>(), int NT = kTileThreads>
+// WARM: warm_factors before the substitution (single-spline launches).
+template <int BW, bool CYC, typename T, typename C, int B = batch_rows<BW, CYC>(), int NT = kTileThreads,
+          bool WARM = false>
 __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first, unsigned char* tile_bytes,
                                            int64_t* dst_base, int64_t* src_base) {
     C* tile = reinterpret_cast<C*>(tile_bytes);
@@ -379,6 +419,7 @@ __device__ __forceinline__ void solve_tile(const SolveParams& sp, int64_t first,
             }
         }
     }
+    if constexpr (WARM) warm_factors<BW, CYC, C, NT>(sp, n);
     __syncthreads();
 
     if (tid < nl) substitute<BW, CYC, C, C, B>(tile + tid * stride, 1, n, sp);
