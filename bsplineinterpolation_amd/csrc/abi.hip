@@ -203,6 +203,9 @@ struct intp_spline_s {
     int mirror_rc = INTP_OK;
     std::string mirror_error;
     std::vector<double> mirror;
+    // refined splines with a gradient table: its differences per dimension
+    // (mirror_diff[d * total + e]), downloaded with the mirror
+    std::vector<double> mirror_diff;
     ~intp_spline_s() {
         // ordered on the release stream before the grid's own release,
         // which runs when `grid` drops its last reference below
@@ -907,12 +910,34 @@ int intp_spline_download_knots(intp_spline s, int32_t d, double* host, int64_t c
 
 namespace {
 
+// The gradient table's differences into s->mirror_diff (FP64).
+int download_differences(intp_spline s) {
+    const int D = s->grid->dim;
+    const int64_t total = s->grid->total;
+    const int per = cell_bytes(D) / static_cast<int>(sizeof(float));  // floats per cell
+    DeviceScope scope(s->grid->device);
+    DeviceRuntime* rt = device_runtime(s->grid->device);
+    if (!rt) return fail(INTP_E_HIP, "cannot create the device's streams");
+    s->uses.order_before(rt->setup);
+    std::vector<float> cells(static_cast<size_t>(total) * per);
+    HIP_TRY(hipMemcpyAsync(cells.data(), s->cells, cells.size() * sizeof(float), hipMemcpyDeviceToHost, rt->setup),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(rt->setup), "hipStreamSynchronize");
+    std::vector<double> diff(static_cast<size_t>(total) * D);
+    for (int d = 0; d < D; ++d)
+        for (int64_t e = 0; e < total; ++e)
+            diff[static_cast<size_t>(d * total + e)] = static_cast<double>(cells[static_cast<size_t>(e * per + 1 + d)]);
+    s->mirror_diff = std::move(diff);
+    return INTP_OK;
+}
+
 // The spline's control points on the host (FP64), downloaded once.
 int ensure_mirror(intp_spline s) {
     if (s->mirror_ready.load(std::memory_order_acquire)) return INTP_OK;  // the per-call path
     std::call_once(s->mirror_once, [s]() {
         std::vector<double> host(static_cast<size_t>(s->grid->total));
         s->mirror_rc = intp_spline_download_ctrl(s, host.data(), s->grid->total);
+        if (s->mirror_rc == INTP_OK && s->cells) s->mirror_rc = download_differences(s);
         if (s->mirror_rc == INTP_OK) {
             s->mirror = std::move(host);
             s->mirror_ready.store(true, std::memory_order_release);
@@ -930,6 +955,8 @@ HostSpline host_view(const intp_spline_s& s) {
     h.order = s.grid->order;
     for (int d = 0; d < h.dim; ++d) h.dims[d] = &s.grid->setup[d];
     h.ctrl = s.mirror.data();
+    h.diff = s.mirror_diff.empty() ? nullptr : s.mirror_diff.data();
+    h.total = s.grid->total;
     return h;
 }
 

@@ -133,26 +133,36 @@ double value_at(const HostSpline& s, const double* x) {
 // Partial derivative, as the reference's derivative_at (BSpline.hpp:393-531):
 // the local (K+1)^D control points in C order, differenced along each
 // differentiated dimension, then sum over C order of (b_0 b_1 b_2) * c.
+// A first partial of a refined spline takes its differences from the
+// gradient table (s.diff) instead of differencing the FP32 points.
 template <int D, int K>
 double derivative_at(const HostSpline& s, const double* x, const int32_t* r) {
     constexpr int W = K + 1;
     constexpr int N = D == 1 ? W : D == 2 ? W * W : W * W * W;
     for (int d = 0; d < D; ++d)
         if (r[d] > K) return 0.0;
+    int first = -1;  // dimension of a first partial read from s.diff
+    if (s.diff) {
+        int sum = 0;
+        for (int d = 0; d < D; ++d) sum += r[d];
+        for (int d = 0; d < D && sum == 1; ++d)
+            if (r[d] == 1) first = d;
+    }
     double b[3][W];
     int64_t off[3][W], mu[3];
     all_terms<D, K>(s, x, r, b, off, mu);
+    const double* src = first >= 0 ? s.diff + first * s.total : s.ctrl;
     double c[N];
     int l = 0;
     if constexpr (D == 1) {
-        for (int i0 = 0; i0 < W; ++i0) c[l++] = s.ctrl[off[0][i0]];
+        for (int i0 = 0; i0 < W; ++i0) c[l++] = src[off[0][i0]];
     } else if constexpr (D == 2) {
         for (int i0 = 0; i0 < W; ++i0)
-            for (int i1 = 0; i1 < W; ++i1) c[l++] = s.ctrl[off[0][i0] + off[1][i1]];
+            for (int i1 = 0; i1 < W; ++i1) c[l++] = src[off[0][i0] + off[1][i1]];
     } else {
         for (int i0 = 0; i0 < W; ++i0)
             for (int i1 = 0; i1 < W; ++i1)
-                for (int i2 = 0; i2 < W; ++i2) c[l++] = s.ctrl[off[0][i0] + off[1][i1] + off[2][i2]];
+                for (int i2 = 0; i2 < W; ++i2) c[l++] = src[off[0][i0] + off[1][i1] + off[2][i2]];
     }
     for (int d = 0; d < D; ++d) {
         if (r[d] == 0) continue;
@@ -166,8 +176,9 @@ double derivative_at(const HostSpline& s, const double* x, const int32_t* r) {
             for (int kk = K; kk > K - r[d]; --kk)
                 for (int j = kk; j > 0; --j) {
                     const int at = K + j - kk;
-                    v[at * lstride] = static_cast<double>(kk) * (v[at * lstride] - v[(at - 1) * lstride]) /
-                                      (t[mu[d] + j] - t[mu[d] + j - kk]);
+                    // gathered differences: v[at] already holds c[at] - c[at - 1]
+                    const double dv = d == first ? v[at * lstride] : v[at * lstride] - v[(at - 1) * lstride];
+                    v[at * lstride] = static_cast<double>(kk) * dv / (t[mu[d] + j] - t[mu[d] + j - kk]);
                 }
         }
     }

@@ -20,6 +20,11 @@ struct HostSpline {
     int dim = 0;
     int order = 0;
     const double* ctrl = nullptr;
+    // refined splines (1 to 3 dimensions): the gradient table's backward
+    // differences, diff[d * total + e] = c[e] - c[e - e_d] formed before
+    // rounding; first partials use them in place of differences of ctrl
+    const double* diff = nullptr;
+    int64_t total = 0;
 };
 
 // Values / derivatives of npts points: coordinate d of point i at

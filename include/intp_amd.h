@@ -162,10 +162,12 @@ int intp_build(intp_ctx ctx, const intp_grid_desc* desc, const double* values, i
  * first partials are evaluated from it, so that the partials carry FP32
  * rounding of the differences rather than cancellation of FP32 control
  * points (about 1e-7 relative instead of 5e-6 with one sweep or more;
- * with 0 sweeps the differences come from the FP32 solve alone).  No
- * reference counterpart: the
- * reference solves in the value type T (InterpolationTemplate.hpp:448-580);
- * this is its mixed-precision variant.                                  */
+ * with 0 sweeps the differences come from the FP32 solve alone).  Host
+ * evaluation (INTP_EVAL_HOST_MIRROR) reads the same differences for first
+ * partials.  Splines of 4 to 6 dimensions keep no gradient table.  No
+ * reference counterpart: the reference solves in the value type T
+ * (InterpolationTemplate.hpp:448-580); this is its mixed-precision
+ * variant.                                                              */
 int intp_interpolate_refined(intp_template t, const double* values, int values_mem, int32_t sweeps,
                              intp_spline* out, void* stream);
 int intp_build_refined(intp_ctx ctx, const intp_grid_desc* desc, const double* values, int values_mem,

@@ -91,11 +91,17 @@ def test_gpu_refined_against_fp64_oracle(amd, orc, shape, k, per, nonuni, data):
         v_max, v_rms = errs(got[0], want[0])
         seen.append((sweeps, c_max, v_max))
         print("sweeps %d: ctrl max %.2e rms %.2e | value max %.2e rms %.2e" % (sweeps, c_max, c_rms, v_max, v_rms))
+        # the host scalar path: the FP32 control points for values, the
+        # gradient table's differences for first partials
+        host = g.eval_host(q, orders)
         for i in range(1, D + 1):
             d_max, d_rms = errs(got[i], want[i])
-            print("  d/dx%d: max %.2e rms %.2e" % (i - 1, d_max, d_rms))
+            h_max, _ = errs(host[i], want[i])
+            print("  d/dx%d: max %.2e rms %.2e | host max %.2e" % (i - 1, d_max, d_rms, h_max))
             if sweeps >= 1:
-                assert d_max <= value_bar, (sweeps, i, d_max, d_rms)
+                assert d_max <= value_bar and h_max <= value_bar, (sweeps, i, d_max, d_rms, h_max)
+        if sweeps >= 1:
+            assert errs(host[0], want[0])[1] <= value_bar
         if sweeps >= 1:
             assert c_max <= BAR and v_rms <= value_bar and v_max <= 5 * value_bar, seen
     # refinement never makes the control points worse than the plain FP32 solve
