@@ -174,6 +174,25 @@ def measure_refined_form(amd, torch, steps: int, warmup: int, n: int, nq: int, s
 SERVE_RATES = (1e2, 3e2, 1e3, 3e3, 1e4)
 
 
+def measure_copy_gbs(torch, nbytes: int = 1 << 31, reps: int = 5) -> float:
+    """The device's streaming bandwidth as torch measures it: a copy of
+    `nbytes` (read + written) timed with events, in GB/s."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def refined_build_bytes(dim: int, N: int, sweeps: int) -> int:
     """Algorithmic HBM bytes of a refined build with S sweeps (as the
     serving loop counts them): FP32 solves, FP64 residuals, the FP32
@@ -775,6 +794,12 @@ def main():
                      "traffic": profiled_traffic(EVAL_KERNEL) if (n, nq) == (256, 1 << 26) else None,
                      "traffic_source": TRAFFIC_PROFILE},
     }
+    # the same figures against the box's measured streaming bandwidth
+    copy_gbs = measure_copy_gbs(torch)
+    result["measured_copy_gbs"] = copy_gbs
+    result["roofline"]["frac_of_measured_copy"] = kernel_gbs / copy_gbs
+    result["solve"]["frac_of_measured_copy"] = solve_gbs / copy_gbs
+    result["eval_call"]["frac_of_measured_copy"] = eval_gbs / copy_gbs
 
     if not args.no_forms:
         print("bench: refined-FP32 form (3D order 5)", file=sys.stderr, flush=True)

@@ -1,5 +1,5 @@
 // kernels.hpp -- launch interface between the C ABI (abi.hip) and the
-// gfx950 kernels (eval.hip, solve.hip).  Plain structs passed by value
+// gfx950 kernels (eval_impl.hpp, solve.hip, bin.hip, ...).  Plain structs passed by value
 // as kernel arguments.
 #pragma once
 
