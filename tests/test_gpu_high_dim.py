@@ -146,3 +146,27 @@ def test_gpu_high_dim_refined(amd, orc):
     want = o.eval(q)[0]
     got = g.eval(q)[0]
     assert float(np.sqrt(np.mean((got - want) ** 2) / np.mean(want ** 2))) < 1e-6
+
+
+def test_gpu_4d_larger_grid(amd, orc):
+    """A 4D grid of 1.4M control points (40 x 36 x 32 x 30, order 3, dims 0
+    and 2 periodic): 2^18 points of value and the four partials on the
+    device, a 3000-point subsample against the oracle."""
+    import torch
+    rng = np.random.default_rng(404)
+    shape = (40, 36, 32, 30)
+    per = [True, False, True, False]
+    ranges = [(0.0, 1.0), (-1.0, 1.0), (0.0, 2.0), (0.5, 1.5)]
+    f = rng.standard_normal(shape)
+    print("4D grid %s, 2^18 points" % (shape,), flush=True)
+    g = amd.InterpolationFunction(3, f, periodic=per, ranges=ranges)
+    o = orc.OracleSpline(3, f, periodic=per, ranges=ranges)
+    assert np.array_equal(g.control_points(), o.control_points())
+    n = 1 << 18
+    q = np.stack([rng.uniform(lo, hi, n) for lo, hi in ranges], axis=1)
+    orders = [[0] * 4] + [list(e) for e in np.eye(4, dtype=int)]
+    got = g.eval(torch.from_numpy(q).cuda(), orders).cpu().numpy()
+    idx = rng.choice(n, 3000, replace=False)
+    want = o.eval(q[idx], orders)
+    for i in range(len(orders)):
+        assert rel_err(got[i][idx], want[i]) < TOL, (orders[i], rel_err(got[i][idx], want[i]))
