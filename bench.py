@@ -609,8 +609,8 @@ def measure_drop_in_timing():
     cmd = [exe] + ([ref] if os.path.exists(ref) else [])
     r = subprocess.run(cmd, capture_output=True, text=True, check=True)
     rows = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
-    return dict(workload="2^15 random points sorted by cell, 32 passes, meshes of 2^18 points (1D 262144, "
-                         "2D 512^2, 3D 64^3), even dimensions periodic; reference operator() on one thread",
+    return dict(workload="2^15 random points sorted by cell, 32 passes, meshes of about 2^18 points (1D 262144, "
+                         "2D 512^2, 3D 64^3, 4D 24^4), even dimensions periodic; reference operator() on one thread",
                 rows=rows)
 
 

@@ -281,84 +281,90 @@ const std::array<std::array<ApplyFn, kMaxK + 1>, 3> kApplyFns = {
     apply_fns<2>(std::make_integer_sequence<int, kMaxK + 1>{}),
     apply_fns<3>(std::make_integer_sequence<int, kMaxK + 1>{})};
 
-// ---- 4 to 6 dimensions: the loops above with the dimension at run time ----
+// ---- 4 to 6 dimensions: the same loops, nested by recursion over the
+// dimensions (D a template argument); the offsets of the outer dimensions
+// are summed once per level ----
 
-// The (K+1)^D stencil as a counter a[0..D-1] with dimension `fastest`
-// varying fastest (0: the reference's cell order; D-1: C order).
-template <int K>
-struct StencilCounter {
-    int a[kHostMaxDim] = {};
-    int dim;
-    bool next_dim0_fastest() {
-        for (int d = 0; d < dim; ++d) {
-            if (++a[d] <= K) return true;
-            a[d] = 0;
-        }
-        return false;
-    }
-    bool next_c_order() {
-        for (int d = dim - 1; d >= 0; --d) {
-            if (++a[d] <= K) return true;
-            a[d] = 0;
-        }
-        return false;
-    }
-};
-
-template <int K>
+template <int D, int K>
 void all_terms_nd(const HostSpline& s, const double* x, const int32_t* r, double (&b)[kHostMaxDim][K + 1],
                   int64_t (&off)[kHostMaxDim][K + 1], int64_t (&mu)[kHostMaxDim]) {
     int64_t stride = 1;
-    for (int d = s.dim - 1; d >= 0; --d) {
+    for (int d = D - 1; d >= 0; --d) {
         mu[d] = dimension_terms<K>(*s.dims[d], stride, x[d], r ? r[d] : 0, b[d], off[d]);
         stride *= s.dims[d]->n;
     }
 }
 
-// value_at for D >= 4: terms ((c * b_0) * b_1) ... in cell order
-template <int K>
+// Cell order, dimension 0 innermost: each term ((c * w_0) * w_1) ... added
+// in turn (value_at's order).
+template <int D, int K, int L>
+inline void cell_sum(const double* c, const double (&w)[kHostMaxDim][K + 1], const int64_t (&off)[kHostMaxDim][K + 1],
+                     int64_t base, int (&a)[kHostMaxDim], double& v) {
+    for (int i = 0; i <= K; ++i) {
+        a[L] = i;
+        if constexpr (L == 0) {
+            double coef = c[base + off[0][i]];
+            coef *= w[0][i];
+            for (int d = 1; d < D; ++d) coef *= w[d][a[d]];
+            v += coef;
+        } else {
+            cell_sum<D, K, L - 1>(c, w, off, base + off[L][i], a, v);
+        }
+    }
+}
+
+// C order, dimension D-1 innermost: the stencil's elements into out.
+template <int D, int K, int L>
+inline void gather_c_order(const double* c, const int64_t (&off)[kHostMaxDim][K + 1], int64_t base, double*& out) {
+    for (int i = 0; i <= K; ++i) {
+        if constexpr (L == D - 1)
+            *out++ = c[base + off[L][i]];
+        else
+            gather_c_order<D, K, L + 1>(c, off, base + off[L][i], out);
+    }
+}
+
+// C-order sum of (b_0 b_1 ...) * c: the product b_0 b_1 .. b_L is formed
+// left to right as in derivative_at, its prefix shared by the terms below
+// a level.
+template <int D, int K, int L>
+inline void c_order_sum(const double (&b)[kHostMaxDim][K + 1], double prefix, const double*& c, double& v) {
+    for (int i = 0; i <= K; ++i) {
+        const double q = L == 0 ? b[0][i] : prefix * b[L][i];
+        if constexpr (L == D - 1)
+            v += q * *c++;
+        else
+            c_order_sum<D, K, L + 1>(b, q, c, v);
+    }
+}
+
+template <int D, int K>
 double value_at_nd(const HostSpline& s, const double* x) {
     double b[kHostMaxDim][K + 1];
     int64_t off[kHostMaxDim][K + 1], mu[kHostMaxDim];
-    all_terms_nd<K>(s, x, nullptr, b, off, mu);
-    StencilCounter<K> it;
-    it.dim = s.dim;
+    all_terms_nd<D, K>(s, x, nullptr, b, off, mu);
+    int a[kHostMaxDim] = {};
     double v = 0.0;
-    do {
-        int64_t o = 0;
-        for (int d = 0; d < s.dim; ++d) o += off[d][it.a[d]];
-        double coef = s.ctrl[o];
-        for (int d = 0; d < s.dim; ++d) coef *= b[d][it.a[d]];
-        v += coef;
-    } while (it.next_dim0_fastest());
+    cell_sum<D, K, D - 1>(s.ctrl, b, off, 0, a, v);
     return v;
 }
 
-// derivative_at for D >= 4: C-order gather, differences along each
-// differentiated dimension, then the C-order sum of (b_0 b_1 ...) * c
-template <int K>
+template <int D, int K>
 double derivative_at_nd(const HostSpline& s, const double* x, const int32_t* r) {
     constexpr int W = K + 1;
-    const int D = s.dim;
+    int64_t total = 1;
+    for (int d = 0; d < D; ++d) total *= W;
     for (int d = 0; d < D; ++d)
         if (r[d] > K) return 0.0;
     double b[kHostMaxDim][K + 1];
     int64_t off[kHostMaxDim][K + 1], mu[kHostMaxDim];
-    all_terms_nd<K>(s, x, r, b, off, mu);
-    int64_t total = 1;
-    for (int d = 0; d < D; ++d) total *= W;
+    all_terms_nd<D, K>(s, x, r, b, off, mu);
     thread_local std::vector<double> buf;
     buf.resize(static_cast<size_t>(total));
     double* c = buf.data();
     {
-        StencilCounter<K> it;
-        it.dim = D;
-        int64_t l = 0;
-        do {
-            int64_t o = 0;
-            for (int d = 0; d < D; ++d) o += off[d][it.a[d]];
-            c[l++] = s.ctrl[o];
-        } while (it.next_c_order());
+        double* out = c;
+        gather_c_order<D, K, 0>(s.ctrl, off, 0, out);
     }
     for (int d = 0; d < D; ++d) {
         if (r[d] == 0) continue;
@@ -377,61 +383,58 @@ double derivative_at_nd(const HostSpline& s, const double* x, const int32_t* r) 
                 }
         }
     }
-    StencilCounter<K> it;
-    it.dim = D;
     double v = 0.0;
-    int64_t l = 0;
-    do {
-        double coef = b[0][it.a[0]];
-        for (int d = 1; d < D; ++d) coef *= b[d][it.a[d]];
-        v += coef * c[l++];
-    } while (it.next_c_order());
+    const double* in = c;
+    c_order_sum<D, K, 0>(b, 1.0, in, v);
     return v;
 }
 
-template <int K>
+template <int D, int K>
 double eval_point_nd(const HostSpline& s, const double* x, const int32_t* r) {
-    for (int d = 0; d < s.dim; ++d)
-        if (r[d] != 0) return derivative_at_nd<K>(s, x, r);
-    return value_at_nd<K>(s, x);
+    for (int d = 0; d < D; ++d)
+        if (r[d] != 0) return derivative_at_nd<D, K>(s, x, r);
+    return value_at_nd<D, K>(s, x);
 }
 
 // apply_weights for D >= 4 (value_at_nd's order)
-template <int K>
+template <int D, int K>
 double apply_weights_nd(const HostSpline& s, const int32_t* first, const double* w) {
     constexpr int W = K + 1;
     int64_t off[kHostMaxDim][W];
+    double wl[kHostMaxDim][W];
     int64_t stride = 1;
-    for (int d = s.dim - 1; d >= 0; --d) {
-        for (int m = 0; m < W; ++m) off[d][m] = stencil_index(*s.dims[d], first[d], m) * stride;
+    for (int d = D - 1; d >= 0; --d) {
+        for (int m = 0; m < W; ++m) {
+            off[d][m] = stencil_index(*s.dims[d], first[d], m) * stride;
+            wl[d][m] = w[d * W + m];
+        }
         stride *= s.dims[d]->n;
     }
-    StencilCounter<K> it;
-    it.dim = s.dim;
+    int a[kHostMaxDim] = {};
     double v = 0.0;
-    do {
-        int64_t o = 0;
-        for (int d = 0; d < s.dim; ++d) o += off[d][it.a[d]];
-        double coef = s.ctrl[o];
-        for (int d = 0; d < s.dim; ++d) coef *= w[d * W + it.a[d]];
-        v += coef;
-    } while (it.next_dim0_fastest());
+    cell_sum<D, K, D - 1>(s.ctrl, wl, off, 0, a, v);
     return v;
 }
 
-template <int... K>
+template <int D, int... K>
 constexpr std::array<PointFn, sizeof...(K)> point_fns_nd(std::integer_sequence<int, K...>) {
-    return {&eval_point_nd<K>...};
+    return {&eval_point_nd<D, K>...};
 }
-template <int... K>
+template <int D, int... K>
 constexpr std::array<ApplyFn, sizeof...(K)> apply_fns_nd(std::integer_sequence<int, K...>) {
-    return {&apply_weights_nd<K>...};
+    return {&apply_weights_nd<D, K>...};
 }
-const std::array<PointFn, kMaxK + 1> kPointFnsNd = point_fns_nd(std::make_integer_sequence<int, kMaxK + 1>{});
-const std::array<ApplyFn, kMaxK + 1> kApplyFnsNd = apply_fns_nd(std::make_integer_sequence<int, kMaxK + 1>{});
+const std::array<std::array<PointFn, kMaxK + 1>, 3> kPointFnsNd = {
+    point_fns_nd<4>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    point_fns_nd<5>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    point_fns_nd<6>(std::make_integer_sequence<int, kMaxK + 1>{})};
+const std::array<std::array<ApplyFn, kMaxK + 1>, 3> kApplyFnsNd = {
+    apply_fns_nd<4>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    apply_fns_nd<5>(std::make_integer_sequence<int, kMaxK + 1>{}),
+    apply_fns_nd<6>(std::make_integer_sequence<int, kMaxK + 1>{})};
 
-PointFn point_fn(int dim, int order) { return dim <= 3 ? kPointFns[dim - 1][order] : kPointFnsNd[order]; }
-ApplyFn apply_fn(int dim, int order) { return dim <= 3 ? kApplyFns[dim - 1][order] : kApplyFnsNd[order]; }
+PointFn point_fn(int dim, int order) { return dim <= 3 ? kPointFns[dim - 1][order] : kPointFnsNd[dim - 4][order]; }
+ApplyFn apply_fn(int dim, int order) { return dim <= 3 ? kApplyFns[dim - 1][order] : kApplyFnsNd[dim - 4][order]; }
 
 }  // namespace
 

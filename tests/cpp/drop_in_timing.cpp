@@ -2,7 +2,7 @@
 // the reference's eval-proxy timing (test/src/interpolation-eval-proxy-test.cpp:
 // 2^15 random coordinates sorted by cell, evaluated 32 times directly and
 // 32 times through eval_proxy objects built once per coordinate).
-// Meshes of about 2^18 points in 1D, 2D and 3D, even dimensions periodic,
+// Meshes of about 2^18 points in 1D to 4D, even dimensions periodic,
 // values a product of cosines; orders 1, 3 and 5.
 //
 // The scalar members run on the calling thread from a host copy of the
@@ -153,6 +153,7 @@ void orders_for_all_dims() {
     run<1, K>(std::make_index_sequence<1>{}, {std::size_t{1} << 18});
     run<2, K>(std::make_index_sequence<2>{}, {512, 512});
     run<3, K>(std::make_index_sequence<3>{}, {64, 64, 64});
+    run<4, K>(std::make_index_sequence<4>{}, {24, 24, 24, 24});
 }
 
 }  // namespace
