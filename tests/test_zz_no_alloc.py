@@ -66,6 +66,26 @@ def test_gpu_eval_static_equals_eval(amd):
     assert rc == 0 and torch.equal(got, _call(amd, "intp_eval", g, small, orders[:1])[1])
 
 
+def test_gpu_eval_static_refined(amd):
+    """A refined spline: value and partials come from its gradient table,
+    through the fixed buffer as through intp_eval (grouped and not, two
+    calls in a row, the second smaller)."""
+    import torch
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((96, 80, 72))
+    g = amd.InterpolationFunction(3, f, periodic=[True, False, True], ranges=[(0, 1)] * 3, precision="f32_refined",
+                                  sweeps=1)
+    orders = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]
+    big = torch.from_numpy(rng.uniform(0, 1, ((1 << 19) + 5, 3))).cuda()
+    small = torch.from_numpy(rng.uniform(0, 1, (777, 3))).cuda()
+    for flags in (0, amd._lib.INTP_EVAL_GROUPED):
+        for pts in (big, small):
+            rc, got = _call(amd, "intp_eval_static", g, pts, orders, flags)
+            assert rc == 0, amd._lib.load().intp_last_error()
+            rc, want = _call(amd, "intp_eval", g, pts, orders, flags)
+            assert rc == 0 and torch.equal(got, want), (flags, pts.shape)
+
+
 def test_gpu_eval_static_limits(amd):
     import torch
     lib = amd._lib.load()
