@@ -158,6 +158,73 @@ auto get_range(C& c) -> std::pair<decltype(std::begin(c)), decltype(std::end(c))
     return std::make_pair(std::begin(c), std::end(c));
 }
 
+// The helpers of the reference's util.hpp that user code can name
+// (util.hpp:18-48, 150-198, 278-303), on the standard library.
+template <std::size_t... I>
+using index_sequence = std::index_sequence<I...>;
+template <std::size_t N>
+using make_index_sequence = std::make_index_sequence<N>;
+template <typename... T>
+using make_index_sequence_for = std::index_sequence_for<T...>;
+
+// base^exp for an unsigned exponent, multiplied as base * (base * (... * 1)).
+template <typename T1, typename T2>
+constexpr typename std::enable_if<std::is_unsigned<T2>::value, T1>::type pow(T1 base, T2 exp) {
+    T1 r{1};
+    for (T2 i = 0; i < exp; ++i) r = base * r;
+    return r;
+}
+
+template <typename T>
+using remove_cvref_t = typename std::remove_cv<typename std::remove_reference<T>::type>::type;
+
+namespace detail_traits {
+template <typename It>
+using input_iterator_like = std::is_convertible<typename std::iterator_traits<It>::iterator_category,
+                                                std::input_iterator_tag>;
+template <typename T, typename = void>
+struct iterable : std::false_type {};
+template <typename T>
+struct iterable<T, typename std::enable_if<input_iterator_like<decltype(std::declval<T&>().begin())>::value &&
+                                           input_iterator_like<decltype(std::declval<T&>().end())>::value>::type>
+    : std::true_type {};
+template <typename T, typename = void>
+struct indexed : std::false_type {};
+template <typename T>
+struct indexed<T, decltype(void(std::declval<const T&>().operator[](0)))> : std::true_type {};
+}  // namespace detail_traits
+
+// T has begin() and end() returning input iterators.
+template <typename T>
+struct is_iterable {
+    static constexpr bool value = detail_traits::iterable<T>::value;
+};
+
+// const T has a member operator[](integer).
+template <typename T>
+struct is_indexed {
+    static constexpr bool value = detail_traits::indexed<T>::value;
+};
+
+// An allocator that default-initializes (leaves trivial elements
+// uninitialized) where a container would value-initialize.
+template <typename T, typename A = std::allocator<T>>
+struct default_init_allocator : public A {
+    template <typename U>
+    struct rebind {
+        using other = default_init_allocator<U, typename std::allocator_traits<A>::template rebind_alloc<U>>;
+    };
+    using A::A;
+    template <typename U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U* p, Args&&... args) {
+        std::allocator_traits<A>::construct(static_cast<A&>(*this), p, std::forward<Args>(args)...);
+    }
+};
+
 }  // namespace util
 
 // ---- Mesh ------------------------------------------------------------------

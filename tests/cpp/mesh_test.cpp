@@ -1,13 +1,15 @@
 // The drop-in's Mesh / MeshDimension surface against the reference's
 // semantics (src/include/Mesh.hpp): C-order indexing, dim_acc_size,
 // iterators along one dimension (skip iterators) and their indices,
-// resize, allocator-aware construction.  Host-only: needs no GPU and no
-// library.  Prints one line per check; exits non-zero on a failure.
+// resize, allocator-aware construction; and the util helpers
+// (src/include/util.hpp).  Host-only: needs no GPU and no library.  Prints one line per check; exits non-zero on a failure.
 #include <intp/Interpolation.hpp>
 
 #include <cstdio>
+#include <initializer_list>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -34,7 +36,29 @@ struct CountingAlloc {
 
 }  // namespace
 
+// The reference's util-test.cpp assertions on the helpers user code can
+// name (index sequences, pow, the iterable / indexed traits), plus the
+// default-initializing allocator.
+void util_checks() {
+    namespace u = intp::util;
+    expect("util::make_index_sequence", std::is_same<u::make_index_sequence<3>, u::index_sequence<0u, 1u, 2u>>::value);
+    expect("util::make_index_sequence_for",
+           std::is_same<u::make_index_sequence_for<int, char>, u::index_sequence<0u, 1u>>::value);
+    static_assert(u::pow(2, 3u) == 8, "compile-time pow");
+    expect("util::pow", u::pow(2, 3u) == 8 && u::pow(1.5, 0u) == 1.0 && u::pow(3.0, 2u) == 9.0);
+    expect("util::is_iterable", u::is_iterable<std::vector<int>>::value && !u::is_iterable<double>::value);
+    expect("util::is_indexed",
+           u::is_indexed<std::vector<int>>::value && !u::is_indexed<std::initializer_list<int>>::value &&
+               !u::is_indexed<double*>::value);
+    expect("util::remove_cvref_t", std::is_same<u::remove_cvref_t<const int&>, int>::value);
+    std::vector<double, u::default_init_allocator<double>> v(4, 2.5);
+    v.resize(6);
+    v[5] = 1.0;
+    expect("util::default_init_allocator", v.size() == 6 && v[3] == 2.5 && v[5] == 1.0);
+}
+
 int main() {
+    util_checks();
     intp::MeshDimension<3> md(4, 5, 6);
     expect("size", md.size() == 120);
     expect("dim_acc_size", md.dim_acc_size(0) == 1 && md.dim_acc_size(1) == 6 && md.dim_acc_size(2) == 30 &&
