@@ -60,6 +60,23 @@
 
 #include "../intp_amd.h"
 
+// The reference's assertion switch and macro (util.hpp:247-258): with
+// INTP_ENABLE_ASSERTION (implied by INTP_DEBUG), INTP_ASSERT(cond, msg)
+// throws std::runtime_error(msg) when cond is false; otherwise it is empty.
+#if defined(INTP_DEBUG) && !defined(INTP_ENABLE_ASSERTION)
+#define INTP_ENABLE_ASSERTION
+#endif
+#ifndef INTP_ASSERT
+#ifdef INTP_ENABLE_ASSERTION
+#define INTP_ASSERT(assertion, msg)                                 \
+    do {                                                            \
+        if (!(assertion)) throw std::runtime_error(msg);            \
+    } while (0)
+#else
+#define INTP_ASSERT(assertion, msg)
+#endif
+#endif
+
 namespace intp {
 
 // ---- errors and contexts -------------------------------------------------
@@ -269,7 +286,7 @@ class MeshDimension {
         return indexing_safe(index_type{static_cast<size_type>(ind)...});
     }
     size_type indexing_safe(const index_type& ind) const {
-#if defined(INTP_ENABLE_ASSERTION) || defined(INTP_DEBUG)
+#ifdef INTP_ENABLE_ASSERTION
         for (size_type d = D; d-- > 0;)
             if (ind[d] >= dim_size_[d])
                 throw std::runtime_error("Mesh access out of range at dim " + std::to_string(d) + ", index " +

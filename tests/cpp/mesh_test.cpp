@@ -9,6 +9,7 @@
 #include <initializer_list>
 #include <memory>
 #include <numeric>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -55,6 +56,17 @@ void util_checks() {
     v.resize(6);
     v[5] = 1.0;
     expect("util::default_init_allocator", v.size() == 6 && v[3] == 2.5 && v[5] == 1.0);
+    bool thrown = false;
+    try {
+        INTP_ASSERT(v.size() == 5, "INTP_ASSERT fired");
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+#ifdef INTP_ENABLE_ASSERTION
+    expect("INTP_ASSERT throws with assertions on", thrown);
+#else
+    expect("INTP_ASSERT is empty with assertions off", !thrown);
+#endif
 }
 
 int main() {
