@@ -14,7 +14,8 @@
 //  * operator(), at, derivative and derivative_at in all three argument
 //    forms, Interpolation.hpp:132-244; periodicity / uniform / range /
 //    spline / get_order, :248-267; eval_proxy, :493-506;
-//  * intp::BSpline<T, D, O, U> as a read-only view (BSpline.hpp:32-760);
+//  * intp::BSpline<T, D, O, U> from explicit knots and control points
+//    (BSpline.hpp:188-216) and as a read-only view (BSpline.hpp:32-760);
 //  * intp::InterpolationFunctionTemplate<T, D, O, U> with interpolate() and
 //    eval_proxy (InterpolationTemplate.hpp:32-176), the 1D classes
 //    InterpolationFunction1D (Interpolation.hpp:509-540) and
@@ -501,11 +502,14 @@ inline ProxyHandle own(intp_proxy p) {
 
 // ---- BSpline -----------------------------------------------------------
 //
-// Read-only view of the spline inside an InterpolationFunction (the
-// reference's BSpline<T, D, O, U>, BSpline.hpp:32-760, as returned by
-// InterpolationFunction::spline(), Interpolation.hpp:264): knot vectors,
-// control points, range, periodicity and order, and evaluation.  Knots and
-// control points are copied from the device on first access.
+// The reference's BSpline<T, D, O, U> (BSpline.hpp:32-760): built from
+// explicit knot vectors and control points, or the read-only view of the
+// spline inside an InterpolationFunction (InterpolationFunction::spline(),
+// Interpolation.hpp:264): knot vectors, control points, range,
+// periodicity and order, and evaluation.  Knots and control points are
+// copied from the device on first access.  The reference's in-place
+// loaders (load_knots, load_ctrlPts) have no counterpart: a device table
+// is immutable once built.
 // control_points() is the plain C-order table (the reference's
 // INTP_CELL_LAYOUT build returns its redundant cell-layout copy instead).
 template <typename T, std::size_t D, std::size_t O, typename U = double>
@@ -525,8 +529,50 @@ class BSpline {
     using KnotContainer = std::vector<knot_type>;
     using knot_const_iterator = typename KnotContainer::const_iterator;
     using control_point_type = Mesh<val_type, D>;
+    using ControlPointContainer = control_point_type;
 
     BSpline() = default;
+
+    // A spline from explicit knot vectors and control points
+    // (BSpline.hpp:188-216): knot range d holds n[d] + order + 1 values,
+    // plus order more when dimension d is periodic; an inconsistent count
+    // throws std::runtime_error (the reference's INTP_ASSERT, :203-209).
+    // The control points are copied to the device once, as FP64.
+    template <typename... InputIters>
+    BSpline(DimArray<bool> periodicity, const ControlPointContainer& ctrl_pts,
+            std::pair<InputIters, InputIters>... knot_iter_pairs) {
+        static_assert(sizeof...(InputIters) == D, "one knot range per dimension");
+        const std::array<std::vector<double>, D> knots{
+            std::vector<double>(knot_iter_pairs.first, knot_iter_pairs.second)...};
+        std::array<std::int32_t, D> per{};
+        std::array<std::int64_t, D> n{}, nk{};
+        std::array<const double*, D> kp{};
+        for (size_type d = 0; d < D; ++d) {
+            per[d] = periodicity[d] ? 1 : 0;
+            n[d] = static_cast<std::int64_t>(ctrl_pts.dim_size(d));
+            nk[d] = static_cast<std::int64_t>(knots[d].size());
+            kp[d] = knots[d].data();
+        }
+        std::array<detail::SplineHandle, C> splines;
+        std::vector<double> comp;
+        comp.reserve(ctrl_pts.size());
+        for (std::size_t c = 0; c < C; ++c) {
+            comp.clear();
+            for (auto it = ctrl_pts.begin(); it != ctrl_pts.end(); ++it)
+                comp.push_back(static_cast<double>(traits::get(*it, c)));
+            intp_spline s = nullptr;
+            detail::check(intp_spline_from_knots(detail::context(), static_cast<std::int32_t>(D),
+                                                 static_cast<std::int32_t>(O), per.data(), n.data(), kp.data(),
+                                                 nk.data(), comp.data(), &s));
+            splines[c] = detail::own(s);
+        }
+        intp_spline_info info{};
+        detail::check(intp_spline_get_info(splines[0].get(), &info));
+        bind(splines, info);
+    }
+    template <typename... InputIters>
+    BSpline(const ControlPointContainer& ctrl_pts, std::pair<InputIters, InputIters>... knot_iter_pairs)
+        : BSpline(DimArray<bool>{}, ctrl_pts, knot_iter_pairs...) {}
 
     inline knot_const_iterator knots_begin(size_type d) const { return host().knots[d].cbegin(); }
     inline knot_const_iterator knots_end(size_type d) const { return host().knots[d].cend(); }

@@ -6,7 +6,9 @@ The known-answer cases of the reference's interpolation test
 tests/golden/mma_vectors.json) are written out as C++ blocks that build
 each function through the reference-style constructors and evaluate it
 through operator(), at and the three derivative_at argument forms; the
-test program includes them.
+explicit-knot cases of test/src/bspline-test.cpp become blocks that build
+BSpline<double, D, 3> from knot ranges and control points.  The test
+program includes them.
 
     python tests/cpp/build_drop_in.py [OUT_DIR]   (default tests/cpp/_build)
 """
@@ -20,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(HERE))
 
-from cases import interp_cases  # noqa: E402
+from cases import bspline_cases, interp_cases  # noqa: E402
 
 
 def _num(x: float) -> str:
@@ -106,11 +108,51 @@ def _case_block(c) -> str:
     return "\n".join(lines)
 
 
+def _bspline_block(c) -> str:
+    """One explicit-knot case: BSpline<double, D, 3> built from knot ranges
+    and a control-point mesh (the reference's bspline-test.cpp form),
+    evaluated through operator() or derivative_at."""
+    ctrl = c["ctrl"]
+    D = ctrl.ndim
+    O = c["order"]
+    per = [bool(p) for p in c["periodic"]]
+    pts = c["points"]
+    r = c["orders"][0]
+    T = "intp::BSpline<double, %d, %d>" % (D, O)
+    lines = ["    {  // %s: %s" % (c["name"], c["cite"])]
+    for d, k in enumerate(c["knots"]):
+        lines.append("        static const std::vector<double> knots%d = %s;" % (d, _arr(k)))
+    lines.append("        static const double cp_data[] = %s;" % _arr(ctrl.ravel()))
+    lines.append("        %s::ControlPointContainer cp(%s);" % (T, ", ".join(str(s) for s in ctrl.shape)))
+    lines.append("        std::copy(cp_data, cp_data + %d, cp.begin());" % ctrl.size)
+    ks = ", ".join("std::make_pair(knots%d.begin(), knots%d.end())" % (d, d) for d in range(D))
+    if any(per):
+        lines.append("        const %s sp({%s}, cp, %s);" % (T, ", ".join("true" if p else "false" for p in per), ks))
+    else:
+        lines.append("        const %s sp(cp, %s);" % (T, ks))
+    lines.append("        static const double pts[%d][%d] = {%s};"
+                 % (len(pts), D, ", ".join(_arr(p) for p in pts)))
+    lines.append("        static const double want[%d] = %s;" % (len(pts), _arr(c["expected"])))
+    lines.append("        std::vector<double> got(%d);" % len(pts))
+    lines.append("        for (std::size_t i = 0; i < %d; ++i) {" % len(pts))
+    lines.append("            const std::array<double, %d> x{%s};" % (D, ", ".join("pts[i][%d]" % d for d in range(D))))
+    if any(r):
+        call = "sp.derivative_at({%s})" % ", ".join("std::make_pair(x[%d], %d)" % (d, r[d]) for d in range(D))
+    else:
+        call = "sp(x)"
+    lines.append("            got[i] = %s;" % call)
+    lines.append("        }")
+    lines.append('        report("%s", got.data(), want, %d, %r);' % (c["name"], len(pts), c["tol"]))
+    lines.append("    }")
+    return "\n".join(lines)
+
+
 def generate(out_dir: str) -> str:
     os.makedirs(out_dir, exist_ok=True)
     path = os.path.join(out_dir, "mma_cases.inc")
     body = ["// Generated by tests/cpp/build_drop_in.py from tests/golden/mma_vectors.json."]
     body += [_case_block(c) for c in interp_cases()]
+    body += [_bspline_block(c) for c in bspline_cases()]
     text = "\n".join(body) + "\n"
     if not os.path.exists(path) or open(path).read() != text:
         with open(path, "w") as fh:

@@ -1,5 +1,5 @@
-// C++ drop-in test: the reference's interpolation known answers (generated
-// blocks, mma_cases.inc) plus the behaviour of the class surface that the
+// C++ drop-in test: the reference's interpolation and explicit-knot
+// BSpline known answers (generated blocks, mma_cases.inc) plus the behaviour of the class surface that the
 // reference's tests rely on -- std::domain_error from at(), templates
 // reused over several meshes, eval_proxy, value semantics of copies.
 // Runs on a GPU through libintp_amd.so; prints one line per check and
@@ -271,6 +271,35 @@ void high_dim_checks() {
     report("4D d/dx1 = product with 1D derivative", got_d.data(), want_d.data(), got_d.size(), 1e-12);
 }
 
+// The explicit-knot BSpline constructors in the reference test's own
+// spelling (bspline-test.cpp:49-58): a 1D control mesh from an iterator
+// pair, the knots as an initializer list; a knot count that does not
+// match the control points throws.
+void bspline_checks() {
+    using Spline = intp::BSpline<double, 1, 3>;
+    const auto knots = {0., 0., 0., 0., .5, 1., 1., 1., 1.};
+    const auto cp = {1., 16. / 3, -16. / 3, 4. / 3, 3.};
+    const Spline sp(Spline::ControlPointContainer(std::make_pair(cp.begin(), cp.end())),
+                    std::make_pair(knots.begin(), knots.end()));
+    // the segment's end points interpolate the end control points
+    expect("BSpline knots: value at 0 = first ctrl", std::abs(sp({0.0}) - 1.0) < 1e-15);
+    expect("BSpline knots: value at 1 = last ctrl", std::abs(sp({1.0}) - 3.0) < 1e-15);
+    expect("BSpline knots: range", sp.range(0).first == 0.0 && sp.range(0).second == 1.0);
+    expect("BSpline knots: knots round trip",
+           std::equal(knots.begin(), knots.end(), sp.knots_begin(0), sp.knots_end(0)));
+    expect("BSpline knots: control points round trip",
+           std::equal(cp.begin(), cp.end(), sp.control_points().begin()));
+    bool threw = false;
+    try {
+        const auto short_knots = {0., 0., 0., 0., 1., 1., 1., 1.};
+        const Spline bad(Spline::ControlPointContainer(std::make_pair(cp.begin(), cp.end())),
+                         std::make_pair(short_knots.begin(), short_knots.end()));
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    expect("BSpline knots: count mismatch throws", threw);
+}
+
 }  // namespace
 
 int main() {
@@ -278,6 +307,7 @@ int main() {
     surface_checks();
     value_type_checks();
     high_dim_checks();
+    bspline_checks();
     std::printf("%d of %d checks failed\n", failures, checks);
     return failures == 0 ? 0 : 1;
 }
