@@ -26,8 +26,11 @@
 //  * exceptions: std::domain_error for at()/derivative_at() out of range,
 //    std::bad_alloc when device memory runs out, std::runtime_error for
 //    invalid arguments and HIP errors.
-// Conventions: INTP_PERIODIC_NO_DUMMY_POINT (a periodic dimension stores n
-// values, spacing (max - min) / n) and INTP_CELL_LAYOUT semantics.
+// Conventions: the reference's periodic value layouts, chosen by the same
+// macro -- with INTP_PERIODIC_NO_DUMMY_POINT a periodic dimension of a
+// value mesh holds n values (spacing (max - min) / n), without it n + 1
+// whose last, the dummy point, is ignored -- and INTP_CELL_LAYOUT
+// semantics.
 // Values: double (the FP64 path), float (the refined-FP32 form), or a
 // fixed-size vector of either (component splines on one grid);
 // coordinates double or float.
@@ -706,6 +709,18 @@ class InterpolationFunction {
     // refinement (intp_build_refined, this many sweeps).
     static constexpr std::int32_t refine_sweeps = 1;
 
+    // The periodic value layout (Interpolation.hpp:329-334, 379-388;
+    // InterpolationTemplate.hpp:255-265, 463-486): without
+    // INTP_PERIODIC_NO_DUMMY_POINT a periodic dimension of a value mesh
+    // holds n + 1 values, the last (the dummy point at the period's end)
+    // ignored; with it, n values.  The grid, and the library below, always
+    // take the n values.
+#ifdef INTP_PERIODIC_NO_DUMMY_POINT
+    static constexpr bool dummy_point = false;
+#else
+    static constexpr bool dummy_point = true;
+#endif
+
     // 1D: (periodic, values range, (xmin, xmax)), Interpolation.hpp:49-79
     template <typename InputIter, typename C1, typename C2,
               typename std::enable_if<D == 1 && detail::is_iterator<InputIter>::value>::type* = nullptr>
@@ -728,7 +743,7 @@ class InterpolationFunction {
         detail::describe_all<D>(desc, keep, std::index_sequence_for<Ts...>{}, x_ranges...);
         std::array<detail::SplineHandle, C> splines;
         if (C == 1) {
-            const std::vector<double> values = component(f_mesh, 0);
+            const std::vector<double> values = component(f_mesh, 0, periodicity);
             intp_spline s = nullptr;
             detail::check(refined() ? intp_build_refined(detail::context(), &desc, values.data(), INTP_MEM_HOST,
                                                          refine_sweeps, &s, nullptr)
@@ -739,7 +754,7 @@ class InterpolationFunction {
             intp_template t = nullptr;
             detail::check(intp_template_create(detail::context(), &desc, &t));
             const detail::TemplateHandle th(t, [](intp_template p) { intp_template_destroy(p); });
-            for (std::size_t c = 0; c < C; ++c) splines[c] = interpolate_component(th.get(), f_mesh, c);
+            for (std::size_t c = 0; c < C; ++c) splines[c] = interpolate_component(th.get(), f_mesh, c, periodicity);
         }
         adopt(splines);
     }
@@ -867,21 +882,34 @@ class InterpolationFunction {
         desc.order = static_cast<std::int32_t>(O);
         for (size_type d = 0; d < D; ++d) {
             desc.periodic[d] = periodicity[d] ? 1 : 0;
-            desc.n[d] = static_cast<std::int64_t>(dims.dim_size(d));
+            const size_type n = dims.dim_size(d);
+            desc.n[d] = static_cast<std::int64_t>(dummy_point && periodicity[d] && n > 0 ? n - 1 : n);
         }
         return desc;
     }
 
-    // Component c of a value mesh, as doubles (C order).
-    static std::vector<double> component(const Mesh<T, D>& mesh, std::size_t c) {
+    // Component c of a value mesh, as doubles (C order), without the dummy
+    // points of periodic dimensions.
+    static std::vector<double> component(const Mesh<T, D>& mesh, std::size_t c, const DimArray<bool>& periodicity) {
         std::vector<double> v;
         v.reserve(mesh.size());
-        for (auto it = mesh.begin(); it != mesh.end(); ++it) v.push_back(static_cast<double>(traits::get(*it, c)));
+        DimArray<size_type> idx{};
+        for (auto it = mesh.begin(); it != mesh.end(); ++it) {
+            bool keep = true;
+            if (dummy_point)
+                for (size_type d = 0; d < D; ++d) keep = keep && !(periodicity[d] && idx[d] + 1 == mesh.dim_size(d));
+            if (keep) v.push_back(static_cast<double>(traits::get(*it, c)));
+            for (size_type d = D; d-- > 0;) {  // C order: the last index runs fastest
+                if (++idx[d] < mesh.dim_size(d)) break;
+                idx[d] = 0;
+            }
+        }
         return v;
     }
 
-    static detail::SplineHandle interpolate_component(intp_template t, const Mesh<T, D>& mesh, std::size_t c) {
-        const std::vector<double> values = component(mesh, c);
+    static detail::SplineHandle interpolate_component(intp_template t, const Mesh<T, D>& mesh, std::size_t c,
+                                                      const DimArray<bool>& periodicity) {
+        const std::vector<double> values = component(mesh, c, periodicity);
         intp_spline s = nullptr;
         detail::check(refined() ? intp_interpolate_refined(t, values.data(), INTP_MEM_HOST, refine_sweeps, &s, nullptr)
                                 : intp_interpolate(t, values.data(), INTP_MEM_HOST, &s, nullptr));
@@ -950,8 +978,10 @@ class InterpolationFunctionTemplate {
     // knots and factored collocation matrices of a grid,
     // InterpolationTemplate.hpp:60-79
     template <typename... Ts>
+    // Without INTP_PERIODIC_NO_DUMMY_POINT, mesh_dimension counts the dummy
+    // point of each periodic dimension, as the meshes given to interpolate().
     InterpolationFunctionTemplate(DimArray<bool> periodicity, MeshDim mesh_dimension, std::pair<Ts, Ts>... x_ranges)
-        : dimension_(mesh_dimension) {
+        : dimension_(mesh_dimension), periodicity_(periodicity) {
         static_assert(sizeof...(Ts) == D, "one coordinate range per dimension");
         intp_grid_desc desc = function_type::describe(periodicity, mesh_dimension);
         std::vector<std::vector<double>> keep(D);
@@ -983,7 +1013,7 @@ class InterpolationFunctionTemplate {
                 throw std::runtime_error("value mesh shape does not match the template");
         std::array<detail::SplineHandle, detail::value_traits<T>::components> splines;
         for (std::size_t c = 0; c < splines.size(); ++c)
-            splines[c] = function_type::interpolate_component(template_.get(), mesh, c);
+            splines[c] = function_type::interpolate_component(template_.get(), mesh, c, periodicity_);
         function_type f;
         f.adopt(splines);
         return f;
@@ -996,8 +1026,9 @@ class InterpolationFunctionTemplate {
     void interpolate(function_type& interp, MeshOrIterPair&& mesh_or_range) const {
         interp = interpolate(std::forward<MeshOrIterPair>(mesh_or_range));
     }
-    // Device-resident values (C-order doubles, scalar double splines),
-    // asynchronous on `stream`.
+    // Device-resident values (C-order doubles, scalar double splines,
+    // n values per periodic dimension whatever the dummy-point
+    // convention), asynchronous on `stream`.
     function_type interpolate_device(const double* values, void* stream = nullptr) const {
         static_assert(detail::value_traits<T>::components == 1, "device values are scalar");
         intp_spline s = nullptr;
@@ -1019,13 +1050,14 @@ class InterpolationFunctionTemplate {
 
    private:
     MeshDim dimension_;
+    DimArray<bool> periodicity_{};
     detail::TemplateHandle template_;
 };
 
 // ---- 1D conveniences -------------------------------------------------------
 
-// x range [0, len - 1], or [0, len] for a periodic spline (no dummy
-// point), Interpolation.hpp:509-540
+// x range [0, len - 1], or [0, len] for a periodic spline under
+// INTP_PERIODIC_NO_DUMMY_POINT (Interpolation.hpp:509-540)
 template <std::size_t O = std::size_t{3}, typename T = double, typename U = double>
 class InterpolationFunction1D : public InterpolationFunction<T, std::size_t{1}, O, U> {
     using base = InterpolationFunction<T, std::size_t{1}, O, U>;
@@ -1036,7 +1068,7 @@ class InterpolationFunction1D : public InterpolationFunction<T, std::size_t{1}, 
         : InterpolationFunction1D(
               std::make_pair(typename base::coord_type{},
                              static_cast<typename base::coord_type>(std::distance(f_range.first, f_range.second) -
-                                                                    (periodicity ? 0 : 1))),
+                                                                    (periodicity && !base::dummy_point ? 0 : 1))),
               f_range, periodicity) {}
 
     template <typename C1, typename C2, typename InputIter>

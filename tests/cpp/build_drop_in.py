@@ -18,6 +18,8 @@ import os
 import subprocess
 import sys
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(HERE))
@@ -44,12 +46,26 @@ def _case_block(c) -> str:
     r = c["orders"][0]
     deriv = any(r)
     lines = ["    {  // %s: %s" % (c["name"], c["cite"])]
-    lines.append("        static const double f_data[] = %s;" % _arr(f.ravel()))
+    if any(per):
+        # without INTP_PERIODIC_NO_DUMMY_POINT a periodic dimension carries
+        # one more value, the dummy point, which the build ignores
+        # (InterpolationTemplate.hpp:463-486): filled with NaN here
+        fd = np.pad(f, [(0, 1) if p else (0, 0) for p in per], constant_values=np.nan)
+        lines.append("#ifdef INTP_PERIODIC_NO_DUMMY_POINT")
+        lines.append("        static const double f_data[] = %s;" % _arr(f.ravel()))
+        lines.append("        const std::array<std::size_t, %d> f_shape{%s};" % (D, ", ".join(str(v) for v in f.shape)))
+        lines.append("#else")
+        lines.append("        static const double f_data[] = %s;" % _arr(fd.ravel()))
+        lines.append("        const std::array<std::size_t, %d> f_shape{%s};" % (D, ", ".join(str(v) for v in fd.shape)))
+        lines.append("#endif")
+    else:
+        lines.append("        static const double f_data[] = %s;" % _arr(f.ravel()))
+        lines.append("        const std::array<std::size_t, %d> f_shape{%s};" % (D, ", ".join(str(v) for v in f.shape)))
     for d in range(D):
         if coords[d] is not None:
             lines.append("        static const std::vector<double> nodes%d = %s;" % (d, _arr(coords[d])))
     if D == 1:
-        lines.append("        const std::vector<double> fv(f_data, f_data + %d);" % f.size)
+        lines.append("        const std::vector<double> fv(f_data, f_data + f_shape[0]);")
         n = f.shape[0]
         if coords[0] is not None:
             xr = "std::make_pair(nodes0.begin(), nodes0.end())"
@@ -64,8 +80,8 @@ def _case_block(c) -> str:
                          "intp::util::get_range(fv), %s);" % (O, _num(ranges[0][0]), _num(ranges[0][1]),
                                                               "true" if per[0] else "false"))
     else:
-        lines.append("        intp::Mesh<double, %d> f(%s);" % (D, ", ".join(str(s) for s in f.shape)))
-        lines.append("        std::copy(f_data, f_data + %d, f.begin());" % f.size)
+        lines.append("        intp::Mesh<double, %d> f(%s);" % (D, ", ".join("f_shape[%d]" % d for d in range(D))))
+        lines.append("        std::copy(f_data, f_data + f.size(), f.begin());")
         rs = []
         for d in range(D):
             if coords[d] is not None:
@@ -160,13 +176,18 @@ def generate(out_dir: str) -> str:
     return path
 
 
-def build(out_dir: str = os.path.join(HERE, "_build")) -> str:
+def build(out_dir: str = os.path.join(HERE, "_build"), no_dummy: bool = True) -> str:
+    """The test program under the reference's test macro set
+    (INTP_PERIODIC_NO_DUMMY_POINT, test/CMakeLists.txt:46), or, with
+    no_dummy=False, under the reference's default periodic layout
+    (drop_in_test_dummy)."""
     generate(out_dir)
-    exe = os.path.join(out_dir, "drop_in_test")
+    exe = os.path.join(out_dir, "drop_in_test" if no_dummy else "drop_in_test_dummy")
     lib_dir = os.path.join(ROOT, "bsplineinterpolation_amd")
-    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror",
-           "-I" + os.path.join(ROOT, "include"), "-I" + out_dir, os.path.join(HERE, "drop_in_test.cpp"),
-           "-o", exe, "-L" + lib_dir, "-lintp_amd", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib_dir, out_dir)]
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror"]
+    cmd += ["-DINTP_PERIODIC_NO_DUMMY_POINT"] if no_dummy else []
+    cmd += ["-I" + os.path.join(ROOT, "include"), "-I" + out_dir, os.path.join(HERE, "drop_in_test.cpp"),
+            "-o", exe, "-L" + lib_dir, "-lintp_amd", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib_dir, out_dir)]
     subprocess.run(cmd, check=True)
     return exe
 
@@ -177,6 +198,7 @@ def build_timing(out_dir: str = os.path.join(HERE, "_build")) -> str:
     exe = os.path.join(out_dir, "drop_in_timing")
     lib_dir = os.path.join(ROOT, "bsplineinterpolation_amd")
     cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-DINTP_PERIODIC_NO_DUMMY_POINT",
            "-I" + os.path.join(ROOT, "include"), os.path.join(HERE, "drop_in_timing.cpp"), "-o", exe,
            "-L" + lib_dir, "-lintp_amd", "-ldl", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(lib_dir, out_dir)]
     subprocess.run(cmd, check=True)

@@ -212,9 +212,14 @@ void value_type_checks() {
 
     // 2D points on a circle, periodic, vector-valued
     // (interpolation-test.cpp:674-703, tolerance 1e-5)
+    // (the closing point repeats the first under the default layout)
     std::vector<Vec2f> pts;
     const std::size_t pts_n = 31;
+#ifdef INTP_PERIODIC_NO_DUMMY_POINT
     for (std::size_t i = 0; i < pts_n; ++i) {
+#else
+    for (std::size_t i = 0; i <= pts_n; ++i) {
+#endif
         const double theta = 2. * M_PI * static_cast<double>(i) / pts_n;
         pts.emplace_back(std::cos(theta), std::sin(theta));
     }

@@ -22,11 +22,20 @@ def test_cpp_drop_in_compiles(tmp_path):
     assert os.path.exists(exe)
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libintp_amd.so not built")
+def test_cpp_drop_in_dummy_layout_compiles(tmp_path):
+    """The same program without INTP_PERIODIC_NO_DUMMY_POINT (the reference's
+    default periodic layout: a dummy point closes each periodic dimension)."""
+    exe = build_drop_in.build(str(tmp_path), no_dummy=False)
+    assert os.path.exists(exe)
+
+
 @pytest.mark.gpu
-def test_cpp_drop_in_runs():
-    exe = os.path.join(HERE, "cpp", "_build", "drop_in_test")
+@pytest.mark.parametrize("name", ["drop_in_test", "drop_in_test_dummy"])
+def test_cpp_drop_in_runs(name):
+    exe = os.path.join(HERE, "cpp", "_build", name)
     if not os.path.exists(exe):
-        exe = build_drop_in.build()
+        exe = build_drop_in.build(no_dummy=name == "drop_in_test")
     print("running", exe, flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout[-4000:])
