@@ -58,6 +58,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -509,10 +510,12 @@ inline ProxyHandle own(intp_proxy p) {
 // explicit knot vectors and control points, or the read-only view of the
 // spline inside an InterpolationFunction (InterpolationFunction::spline(),
 // Interpolation.hpp:264): knot vectors, control points, range,
-// periodicity and order, and evaluation.  Knots and control points are
-// copied from the device on first access.  The reference's in-place
-// loaders (load_knots, load_ctrlPts) have no counterpart: a device table
-// is immutable once built.
+// periodicity and order, evaluation with or without knot-span hints, and
+// pre_calc_coef.  Knots and control points are copied from the device on
+// first access.  Not provided: the in-place loaders (load_knots,
+// load_ctrlPts; a device table is immutable once built), and the basis
+// helper base_spline_value and debug_output, which the reference's
+// classes use internally.
 // control_points() is the plain C-order table (the reference's
 // INTP_CELL_LAYOUT build returns its redundant cell-layout copy instead).
 template <typename T, std::size_t D, std::size_t O, typename U = double>
@@ -533,6 +536,8 @@ class BSpline {
     using knot_const_iterator = typename KnotContainer::const_iterator;
     using control_point_type = Mesh<val_type, D>;
     using ControlPointContainer = control_point_type;
+    using spline_type = BSpline<T, D, O, U>;
+    using BaseSpline = std::array<knot_type, O + 1>;
 
     BSpline() = default;
 
@@ -593,8 +598,16 @@ class BSpline {
     val_type operator()(Coords... x) const {
         return eval(DimArray<double>{static_cast<double>(x)...}, DimArray<size_type>{}, false);
     }
+    // (coordinate, knot-span hint) per dimension (BSpline.hpp:305-368).  The
+    // reference tries the hint before a binary search and finds the same
+    // span either way; the span search here takes no hint.
+    val_type operator()(DimArray<std::pair<knot_type, size_type>> coord_with_hints) const {
+        DimArray<double> x;
+        for (size_type d = 0; d < D; ++d) x[d] = static_cast<double>(coord_with_hints[d].first);
+        return eval(x, DimArray<size_type>{}, false);
+    }
     // mixed partial derivative, (coordinate, order) per dimension
-    // (BSpline.hpp:393-550)
+    // (BSpline.hpp:540-550)
     val_type derivative_at(DimArray<std::pair<knot_type, size_type>> coord_deriv) const {
         DimArray<double> x;
         DimArray<size_type> r;
@@ -603,6 +616,36 @@ class BSpline {
             r[d] = coord_deriv[d].second;
         }
         return eval(x, r, false);
+    }
+    // (coordinate, hint, order) per dimension (BSpline.hpp:393-531)
+    val_type derivative_at(DimArray<std::tuple<knot_type, size_type, size_type>> coord_hint_deriv) const {
+        DimArray<double> x;
+        DimArray<size_type> r;
+        for (size_type d = 0; d < D; ++d) {
+            x[d] = static_cast<double>(std::get<0>(coord_hint_deriv[d]));
+            r[d] = std::get<2>(coord_hint_deriv[d]);
+        }
+        return eval(x, r, false);
+    }
+    // Basis values and stencil offset at a fixed coordinate, computed once;
+    // the callable applies them to the control points of any spline on
+    // the same knots (BSpline.hpp:244-297): a host proxy (intp_proxy_create).
+    auto pre_calc_coef(DimArray<std::pair<knot_type, size_type>> coord_with_hints) const {
+        DimArray<double> x;
+        for (size_type d = 0; d < D; ++d) x[d] = static_cast<double>(coord_with_hints[d].first);
+        intp_proxy p = nullptr;
+        detail::check(intp_proxy_create(nullptr, handle(), 1, x.data(), 1, static_cast<std::int64_t>(D),
+                                        INTP_EVAL_HOST_MEM, &p, nullptr));
+        const detail::ProxyHandle ph = detail::own(p);
+        return [ph](const spline_type& spline) {
+            val_type v{};
+            for (std::size_t c = 0; c < C; ++c) {
+                double out = 0;
+                detail::check(intp_proxy_eval(ph.get(), spline.handle(c), &out, 1, 0u, nullptr));
+                traits::set(v, c, static_cast<typename traits::scalar>(out));
+            }
+            return v;
+        };
     }
 
     // ---- used by InterpolationFunction ---------------------------------

@@ -150,15 +150,25 @@ def _bspline_block(c) -> str:
                  % (len(pts), D, ", ".join(_arr(p) for p in pts)))
     lines.append("        static const double want[%d] = %s;" % (len(pts), _arr(c["expected"])))
     lines.append("        std::vector<double> got(%d);" % len(pts))
-    lines.append("        for (std::size_t i = 0; i < %d; ++i) {" % len(pts))
-    lines.append("            const std::array<double, %d> x{%s};" % (D, ", ".join("pts[i][%d]" % d for d in range(D))))
     if any(r):
         call = "sp.derivative_at({%s})" % ", ".join("std::make_pair(x[%d], %d)" % (d, r[d]) for d in range(D))
+        # the (coordinate, hint, order) form, hint = order as the reference passes
+        alt = ("sp.derivative_at({%s})"
+               % ", ".join("std::make_tuple(x[%d], std::size_t{%d}, std::size_t{%d})" % (d, O, r[d]) for d in range(D)))
+        alt_name = "hint tuples"
     else:
         call = "sp(x)"
+        # precomputed coefficients applied to the spline (bspline-test.cpp:355-366)
+        alt = "sp.pre_calc_coef({%s})(sp)" % ", ".join("std::make_pair(x[%d], std::size_t{%d})" % (d, O) for d in range(D))
+        alt_name = "pre_calc_coef"
+    lines.append("        std::vector<double> alt(%d);" % len(pts))
+    lines.append("        for (std::size_t i = 0; i < %d; ++i) {" % len(pts))
+    lines.append("            const std::array<double, %d> x{%s};" % (D, ", ".join("pts[i][%d]" % d for d in range(D))))
     lines.append("            got[i] = %s;" % call)
+    lines.append("            alt[i] = %s;" % alt)
     lines.append("        }")
     lines.append('        report("%s", got.data(), want, %d, %r);' % (c["name"], len(pts), c["tol"]))
+    lines.append('        report("%s/%s", alt.data(), want, %d, %r);' % (c["name"], alt_name, len(pts), c["tol"]))
     lines.append("    }")
     return "\n".join(lines)
 
