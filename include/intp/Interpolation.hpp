@@ -512,10 +512,10 @@ inline ProxyHandle own(intp_proxy p) {
 // Interpolation.hpp:264): knot vectors, control points, range,
 // periodicity and order, evaluation with or without knot-span hints, and
 // pre_calc_coef.  Knots and control points are copied from the device on
-// first access.  Not provided: the in-place loaders (load_knots,
-// load_ctrlPts; a device table is immutable once built), and the basis
-// helper base_spline_value and debug_output, which the reference's
-// classes use internally.
+// first access.  The loaders (load_knots, load_ctrlPts) build a new device
+// table once everything is loaded; a table is never changed in place.
+// Not provided: the basis helper base_spline_value and debug_output,
+// which the reference's classes use internally.
 // control_points() is the plain C-order table (the reference's
 // INTP_CELL_LAYOUT build returns its redundant cell-layout copy instead).
 template <typename T, std::size_t D, std::size_t O, typename U = double>
@@ -550,8 +550,48 @@ class BSpline {
     BSpline(DimArray<bool> periodicity, const ControlPointContainer& ctrl_pts,
             std::pair<InputIters, InputIters>... knot_iter_pairs) {
         static_assert(sizeof...(InputIters) == D, "one knot range per dimension");
-        const std::array<std::vector<double>, D> knots{
-            std::vector<double>(knot_iter_pairs.first, knot_iter_pairs.second)...};
+        build(periodicity, ctrl_pts,
+              std::array<KnotContainer, D>{KnotContainer(knot_iter_pairs.first, knot_iter_pairs.second)...});
+    }
+    template <typename... InputIters>
+    BSpline(const ControlPointContainer& ctrl_pts, std::pair<InputIters, InputIters>... knot_iter_pairs)
+        : BSpline(DimArray<bool>{}, ctrl_pts, knot_iter_pairs...) {}
+
+    // An empty spline of the given periodicity, filled by load_knots and
+    // load_ctrlPts (BSpline.hpp:178-186, 217-242).  The device table is
+    // built once every knot vector and the control points are loaded, and
+    // again after each later load; range(d) follows load_knots at once.
+    explicit BSpline(DimArray<bool> periodicity) : periodic_(periodicity) {}
+
+    void load_knots(size_type d, KnotContainer knots) {
+        if (d >= D) throw std::runtime_error("knot dimension out of range");
+        if (knots.size() < 2 * O + 2) throw std::runtime_error("too few knots for the spline order");
+        ranges_[d] = std::make_pair(knots[O], knots[knots.size() - O - (2 - O % 2)]);
+        loaded_.knots[d] = std::move(knots);
+        loaded_.have_knots[d] = true;
+        build_loaded();
+    }
+    void load_ctrlPts(const ControlPointContainer& control_points) {
+        loaded_.ctrl = control_points;
+        loaded_.have_ctrl = true;
+        build_loaded();
+    }
+
+    inline knot_const_iterator knots_begin(size_type d) const { return host().knots[d].cbegin(); }
+    inline knot_const_iterator knots_end(size_type d) const { return host().knots[d].cend(); }
+    inline size_type knots_num(size_type d) const { return host().knots[d].size(); }
+    inline const control_point_type& control_points() const { return host().ctrl; }
+    inline const std::pair<knot_type, knot_type>& range(size_type d) const { return ranges_[d]; }
+    inline bool periodicity(size_type d) const { return periodic_[d]; }
+
+   private:
+    // The device table from knot vectors and control points
+    // (intp_spline_from_knots; FP64 control points, one spline per value
+    // component).
+    void build(const DimArray<bool>& periodicity, const ControlPointContainer& ctrl_pts,
+               const std::array<KnotContainer, D>& knots_in) {
+        std::array<std::vector<double>, D> knots;
+        for (size_type d = 0; d < D; ++d) knots[d].assign(knots_in[d].begin(), knots_in[d].end());
         std::array<std::int32_t, D> per{};
         std::array<std::int64_t, D> n{}, nk{};
         std::array<const double*, D> kp{};
@@ -578,16 +618,14 @@ class BSpline {
         detail::check(intp_spline_get_info(splines[0].get(), &info));
         bind(splines, info);
     }
-    template <typename... InputIters>
-    BSpline(const ControlPointContainer& ctrl_pts, std::pair<InputIters, InputIters>... knot_iter_pairs)
-        : BSpline(DimArray<bool>{}, ctrl_pts, knot_iter_pairs...) {}
+    void build_loaded() {
+        if (!loaded_.have_ctrl) return;
+        for (size_type d = 0; d < D; ++d)
+            if (!loaded_.have_knots[d]) return;
+        build(periodic_, loaded_.ctrl, loaded_.knots);
+    }
 
-    inline knot_const_iterator knots_begin(size_type d) const { return host().knots[d].cbegin(); }
-    inline knot_const_iterator knots_end(size_type d) const { return host().knots[d].cend(); }
-    inline size_type knots_num(size_type d) const { return host().knots[d].size(); }
-    inline const control_point_type& control_points() const { return host().ctrl; }
-    inline const std::pair<knot_type, knot_type>& range(size_type d) const { return ranges_[d]; }
-    inline bool periodicity(size_type d) const { return info_.periodic[d] != 0; }
+   public:
     inline constexpr size_type get_order() const { return order; }
 
     // value at coordinates (BSpline.hpp:305-382)
@@ -652,6 +690,7 @@ class BSpline {
     void bind(const std::array<detail::SplineHandle, C>& splines, const intp_spline_info& info) {
         splines_ = splines;
         info_ = info;
+        for (size_type d = 0; d < D; ++d) periodic_[d] = info.periodic[d] != 0;
         for (size_type d = 0; d < D; ++d)
             ranges_[d] = std::make_pair(static_cast<knot_type>(info.range_lo[d]), static_cast<knot_type>(info.range_hi[d]));
         host_ = std::make_shared<HostCopy>();
@@ -685,10 +724,19 @@ class BSpline {
         std::array<KnotContainer, D> knots;
         control_point_type ctrl{MeshDimension<D>()};
     };
+    // what load_knots / load_ctrlPts have received
+    struct Loaded {
+        std::array<KnotContainer, D> knots{};
+        DimArray<bool> have_knots{};
+        control_point_type ctrl{MeshDimension<D>()};
+        bool have_ctrl = false;
+    };
     std::array<detail::SplineHandle, C> splines_{};
     intp_spline_info info_{};
+    DimArray<bool> periodic_{};
     DimArray<std::pair<knot_type, knot_type>> ranges_{};
     std::shared_ptr<HostCopy> host_;
+    Loaded loaded_;
 
     const HostCopy& host() const {
         (void)handle();

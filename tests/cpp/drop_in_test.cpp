@@ -303,6 +303,25 @@ void bspline_checks() {
         threw = true;
     }
     expect("BSpline knots: count mismatch throws", threw);
+
+    // the loader form (BSpline.hpp:178-186, 217-242): periodicity first,
+    // then knots and control points in either order
+    Spline loaded(std::array<bool, 1>{false});
+    loaded.load_knots(0, Spline::KnotContainer(knots.begin(), knots.end()));
+    expect("BSpline loaders: range after load_knots", loaded.range(0).first == 0.0 && loaded.range(0).second == 1.0);
+    loaded.load_ctrlPts(Spline::ControlPointContainer(std::make_pair(cp.begin(), cp.end())));
+    bool same = true;
+    for (int i = 0; i <= 20; ++i) {
+        const double x = i / 20.0;
+        same = same && loaded({x}) == sp({x}) &&
+               loaded.derivative_at({std::make_pair(x, std::size_t{1})}) ==
+                   sp.derivative_at({std::make_pair(x, std::size_t{1})});
+    }
+    expect("BSpline loaders == knot constructor", same);
+    // a later load rebuilds the table
+    const auto cp2 = {2., 16. / 3, -16. / 3, 4. / 3, 5.};
+    loaded.load_ctrlPts(Spline::ControlPointContainer(std::make_pair(cp2.begin(), cp2.end())));
+    expect("BSpline loaders: reload", std::abs(loaded({0.0}) - 2.0) < 1e-15 && std::abs(loaded({1.0}) - 5.0) < 1e-15);
 }
 
 }  // namespace
