@@ -13,7 +13,9 @@
 //               resident splines, after evicting least-recently-used idle
 //               splines if needed; otherwise it stays queued for a later
 //               turn (evaluations of other splines are not held up);
-//   2. poll the events of launched requests; a finished request gets its
+//   2. poll the events of launched requests, oldest first on each stream
+//      (events on one stream complete in order, so polling stops at a
+//      stream's first unfinished request); a finished request gets its
 //      completion time, unpins its spline, and wakes waiting clients;
 //   3. sleep until a submission, or briefly while work is in flight.
 // Builds and evaluations go through intp_build / intp_build_refined /
@@ -140,7 +142,9 @@ struct intp_server_s {
     std::unordered_map<int64_t, Resident> splines;
     std::list<int64_t> lru;  // front: least recently used
     int64_t resident = 0;
-    std::vector<int64_t> inflight;
+    // launched requests per stream, in launch order (= completion order)
+    std::vector<std::deque<int64_t>> inflight;
+    int64_t inflight_n = 0;
     int64_t unfinished = 0;  // submitted, not yet done or failed
 
     // statistics of the current period (mu)
@@ -212,9 +216,10 @@ void intp_server_s::drop(std::unordered_map<int64_t, Resident>::iterator it) {
 // if the in-flight limit is reached.
 // Called and returns with lk held; drops it around the HIP launches.
 int intp_server_s::dispatch(int64_t id, std::unique_lock<std::mutex>& lk) {
-    if (static_cast<int>(inflight.size()) >= max_inflight) return -1;
+    if (inflight_n >= max_inflight) return -1;
     Request& r = *reqs[id];
-    hipStream_t st = streams[next_stream++ % streams.size()];
+    const size_t si = next_stream++ % streams.size();
+    hipStream_t st = streams[si];
     if (r.kind == INTP_REQ_BUILD) {
         if (r.footprint > budget) {
             finish(r, fail(INTP_E_NOMEM, "build larger than the server's HBM budget"));
@@ -313,32 +318,31 @@ int intp_server_s::dispatch(int64_t id, std::unique_lock<std::mutex>& lk) {
         r.done = ev;
     }
     r.state = INTP_REQ_RUNNING;
-    inflight.push_back(id);
+    inflight[si].push_back(id);
+    ++inflight_n;
     return 1;
 }
 
 // Completes launched requests whose events have fired, and frees released
 // splines nothing uses any more (dispatcher, holding mu).
 void intp_server_s::poll_inflight() {
-    size_t keep = 0;
-    for (size_t i = 0; i < inflight.size(); ++i) {
-        const int64_t id = inflight[i];
-        Request& r = *reqs[id];
-        const hipError_t q = hipEventQuery(r.done);
-        if (q == hipErrorNotReady) {
-            inflight[keep++] = id;
-            continue;
+    for (std::deque<int64_t>& q_stream : inflight) {
+        while (!q_stream.empty()) {
+            Request& r = *reqs[q_stream.front()];
+            const hipError_t q = hipEventQuery(r.done);
+            if (q == hipErrorNotReady) break;
+            q_stream.pop_front();
+            --inflight_n;
+            auto it = splines.find(r.spline);
+            if (it != splines.end()) {
+                if (r.kind == INTP_REQ_EVAL)
+                    --it->second.pins;
+                else
+                    it->second.ready = true;
+            }
+            finish(r, q == hipSuccess ? INTP_OK : hip_fail(q, "request"));
         }
-        auto it = splines.find(r.spline);
-        if (it != splines.end()) {
-            if (r.kind == INTP_REQ_EVAL)
-                --it->second.pins;
-            else
-                it->second.ready = true;
-        }
-        finish(r, q == hipSuccess ? INTP_OK : hip_fail(q, "request"));
     }
-    inflight.resize(keep);
     for (auto it = splines.begin(); it != splines.end();) {
         auto cur = it++;
         if (cur->second.released && cur->second.ready && cur->second.pins == 0) drop(cur);
@@ -369,18 +373,18 @@ void intp_server_s::run() {
             else
                 progressed = true;
         }
-        const size_t before = inflight.size();
+        const int64_t before = inflight_n;
         poll_inflight();
-        progressed = progressed || inflight.size() != before;
+        progressed = progressed || inflight_n != before;
         if (!to_release.empty()) {
             lk.unlock();
             for (intp_spline h : to_release) (void)intp_spline_release(h);
             to_release.clear();
             lk.lock();
         }
-        if (stop && queue.empty() && waiting.empty() && inflight.empty()) break;
+        if (stop && queue.empty() && waiting.empty() && inflight_n == 0) break;
         if (!progressed) {
-            if (inflight.empty() && queue.empty() && waiting.empty())
+            if (inflight_n == 0 && queue.empty() && waiting.empty())
                 wake.wait(lk, [&] { return stop || !queue.empty(); });
             else
                 wake.wait_for(lk, std::chrono::microseconds(20));
@@ -415,6 +419,7 @@ int intp_server_create(intp_ctx ctx, const intp_server_config* cfg, intp_server*
         HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
         s->streams.push_back(st);
     }
+    s->inflight.resize(s->streams.size());
     intp_server_s* raw = s.get();
     s->dispatcher = std::thread([raw] { raw->run(); });
     *out = s.release();
