@@ -40,6 +40,11 @@ inline int hip_fail(hipError_t e, const char* what) {
 class DeviceScope {
    public:
     explicit DeviceScope(int device) {
+        // An error an earlier call on this thread left in the runtime's
+        // last-error slot (the caller's, or a release whose failure the
+        // library ignores on purpose) would otherwise be read back by the
+        // first launch check of this call and reported as its own.
+        (void)hipGetLastError();
         ok_ = hipGetDevice(&prev_) == hipSuccess;
         if (ok_ && prev_ != device) ok_ = hipSetDevice(device) == hipSuccess;
     }
