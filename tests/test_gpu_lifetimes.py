@@ -121,3 +121,40 @@ def test_gpu_eval_timings(amd):
     assert t[0][0] < 0.05 and t[0][1] > 0.0  # small batch: no grouping pass
     assert t[1][0] > 0.0 and t[1][1] > 0.0   # large batch, grouping forced: grouped
     assert amd.eval_timings() == []
+
+
+def _hip_runtime():
+    """The HIP runtime torch loaded (the one this library binds to)."""
+    import ctypes
+    import glob
+    import os
+
+    import torch
+    libs = sorted(glob.glob(os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so*")))
+    if not libs:
+        pytest.skip("torch's HIP runtime library not found")
+    return ctypes.CDLL(libs[0])
+
+
+def test_gpu_stale_runtime_error_not_reported(amd, orc):
+    """An error an earlier call left in the HIP runtime's per-thread
+    last-error slot is not reported by the next build or evaluation (each
+    entry point starts from a clean slot)."""
+    import ctypes
+
+    import torch
+    hip = _hip_runtime()
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal((40, 36))
+    q = np.stack([rng.uniform(0.0, 1.0, 4096), rng.uniform(0.0, 1.0, 4096)])
+    for _ in range(2):
+        # hipSetDevice on a device that does not exist fails and changes nothing
+        assert hip.hipSetDevice(ctypes.c_int(1 << 20)) != 0
+        g = amd.InterpolationFunction(3, torch.from_numpy(f).cuda(), periodic=[False, True],
+                                      ranges=[(0.0, 1.0), (0.0, 1.0)])
+        assert hip.hipSetDevice(ctypes.c_int(1 << 20)) != 0
+        out = g.eval(torch.from_numpy(q).cuda(), [[0, 0], [1, 0]]).cpu().numpy()
+    ref = orc.OracleSpline(3, f, periodic=[False, True], ranges=[(0.0, 1.0), (0.0, 1.0)])
+    want = ref.eval(np.ascontiguousarray(q.T), [[0, 0], [1, 0]])
+    for i in range(2):
+        assert rel_err(out[i], want[i]) < 1e-12
