@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void bin_scatter_kernel(BinParams p, cons
             const uint32_t d = PASS == 0 ? (rec.key & 0xFFu) : ((rec.key >> 8) & 0xFFu);
             const uint32_t slot = gbase[d] + (pos - lstart[d]);
             p.rec_out[slot] = rec;
-            if constexpr (PASS == 0) p.key_out[slot] = rec.key;
+            if constexpr (PASS == 0) p.key_out[slot] = static_cast<uint16_t>(rec.key);
         }
         __syncthreads();
     }
@@ -296,14 +296,14 @@ hipError_t launch_binning(BinParams p, uint32_t* hist, uint32_t* offs, uint32_t*
     const dim3 grid(static_cast<unsigned>(p.nblocks)), block(kThreads);
     // pass 0: caller coordinates -> rec_a (and its keys) by low key byte
     p.rec_out = rec_a;
-    p.key_out = keys;
+    p.key_out = reinterpret_cast<uint16_t*>(keys);
     hipLaunchKernelGGL(bin_hist_kernel<0>, grid, block, 0, s, p, hist);
     hipError_t e = exclusive_scan(hist, offs, tile_sums, nh, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bin_scatter_kernel<0>, grid, block, 0, s, p, offs);
     // pass 1: rec_a -> rec_b by high key byte
     p.rec_in = rec_a;
-    p.key_in = keys;
+    p.key_in = reinterpret_cast<const uint16_t*>(keys);
     p.rec_out = rec_b;
     hipLaunchKernelGGL(bin_hist_kernel<1>, grid, block, 0, s, p, hist);
     e = exclusive_scan(hist, offs, tile_sums, nh, s);

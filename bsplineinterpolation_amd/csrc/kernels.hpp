@@ -108,8 +108,10 @@ struct BinParams {
     int64_t npts;
     const BinRecord* rec_in;
     BinRecord* rec_out;
-    const uint32_t* key_in;  // pass 1: keys of rec_in, dense
-    uint32_t* key_out;       // pass 0: keys of rec_out, dense
+    // brick keys (16 bits) of the pass-0 records, dense: written by the
+    // pass-0 scatter, read by the pass-1 histogram
+    const uint16_t* key_in;
+    uint16_t* key_out;
 };
 
 // Evaluation proxies (proxy.hip): create fills first/weights for npts
