@@ -613,8 +613,9 @@ int64_t one_pass_scratch(int64_t npts, int32_t nbins) {
 }
 
 int64_t two_pass_scratch(int64_t npts) {
-    const int64_t nh = bin_hist_entries(npts), nt = bin_scan_tiles(npts);
-    return 2 * ((npts * static_cast<int64_t>(sizeof(BinRecord)) + 255) / 256 * 256) + (2 * nh + nt + npts) * 4 + 256;
+    const int64_t nh = bin_hist_entries(npts), nt = (bin_scan_tiles(npts) + 3) / 4 * 4;
+    return 2 * ((npts * static_cast<int64_t>(sizeof(BinRecord)) + 255) / 256 * 256) + (2 * nh + nt + (npts + 1) / 2) * 4 +
+           256;
 }
 
 // The uniform fast path needs every dimension uniform (knots in closed
@@ -1279,11 +1280,13 @@ int eval_call(intp_spline s, int64_t npts, const double* coords, int64_t cs_dim,
         bp.cs_dim = p.cs_dim;
         bp.cs_pt = p.cs_pt;
         bp.npts = npts;
-        const int64_t nh = bin_hist_entries(npts), nt = bin_scan_tiles(npts);
+        // histograms, their scan, scan tiles, then the 16-bit keys from a
+        // 16-byte boundary (the pass-1 histogram reads them 8 at a time)
+        const int64_t nh = bin_hist_entries(npts), nt = (bin_scan_tiles(npts) + 3) / 4 * 4;
         e = sc.get(static_cast<size_t>(npts) * sizeof(BinRecord), reinterpret_cast<void**>(&rec_a));
         if (e == hipSuccess) e = sc.get(static_cast<size_t>(npts) * sizeof(BinRecord), reinterpret_cast<void**>(&rec_b));
         if (e == hipSuccess)
-            e = sc.get(static_cast<size_t>(2 * nh + nt + npts) * 4, reinterpret_cast<void**>(&bin_scratch));
+            e = sc.get(static_cast<size_t>(2 * nh + nt + (npts + 1) / 2) * 4, reinterpret_cast<void**>(&bin_scratch));
         if (e == hipSuccess)
             e = launch_binning(bp, bin_scratch, bin_scratch + nh, bin_scratch + 2 * nh, bin_scratch + 2 * nh + nt,
                                rec_a, rec_b, stream);

@@ -92,6 +92,24 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
     // (a load per LDS atomic leaves one load in flight per thread)
     double x[kPerThread][kGroupDim];
     uint32_t key[kPerThread];
+    // pass 1: the thread's kPerThread keys are consecutive (one 16-byte
+    // load; the histogram does not depend on which thread counts a key)
+    const int64_t kfirst = first + static_cast<int64_t>(threadIdx.x) * kPerThread;
+    if constexpr (PASS == 1) {
+        static_assert(kPerThread == 8, "one 16-byte load of 16-bit keys");
+        if (kfirst + kPerThread <= p.npts) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p.key_in + kfirst);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                key[2 * j] = w[j] & 0xFFFFu;
+                key[2 * j + 1] = w[j] >> 16;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kPerThread; ++j) key[j] = kfirst + j < p.npts ? p.key_in[kfirst + j] : 0u;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
         const int64_t i = first + j * kThreads + threadIdx.x;
@@ -99,14 +117,12 @@ __global__ __launch_bounds__(kThreads) void bin_hist_kernel(BinParams p, uint32_
 #pragma unroll
             for (int d = 0; d < kGroupDim; ++d) x[j][d] = 0.0;
             if (i < p.npts) load_point(p, i, x[j]);
-        } else {
-            key[j] = i < p.npts ? p.key_in[i] : 0u;
         }
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-        const int64_t i = first + j * kThreads + threadIdx.x;
+        const int64_t i = PASS == 0 ? first + j * kThreads + threadIdx.x : kfirst + j;
         if (i < p.npts) {
             const uint32_t digit = PASS == 0 ? brick_key(p, x[j]) & 0xFFu : (key[j] >> 8) & 0xFFu;
             atomicAdd(&h[digit], 1u);
